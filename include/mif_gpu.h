@@ -1,0 +1,171 @@
+/*
+ * mif_gpu.h -- C ABI of the MI355X (gfx950) incompressible Navier-Stokes
+ * timestep library (libmif_gpu.so).
+ *
+ * The entry points replace the reference solver's single-rank hot path: its
+ * tensors become device fields (mif_field), its domain constants plus
+ * pressure-solver structures become a context (mif_ctx), and each solver
+ * function keeps the reference's arguments and meaning.  Plain pointers and
+ * sizes only; no C++ types cross the boundary.  Every function returns a
+ * mif_status (0 = success) and records a message retrievable with
+ * mif_last_error().
+ *
+ * Host arrays passed to upload/download use the reference tensor layout:
+ * extents (sx, sy, sz) of the staggering, index i + sx*(j + sy*k), FP64
+ * (include/Tensor.h:127-141,219-238).  Device storage is private (padded
+ * pitch; FP32 in the mixed-precision form).
+ *
+ * Solver calls are enqueued on the context's HIP stream and return without
+ * waiting; uploads, downloads, norms and mif_ctx_synchronize() wait.  A
+ * context is not thread-safe; separate contexts may be used from separate
+ * threads.
+ */
+#ifndef MIF_GPU_H
+#define MIF_GPU_H
+
+#include "mif_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mif_ctx mif_ctx;
+typedef struct mif_field mif_field;
+
+/* Storage/arithmetic precision of a context. */
+typedef enum mif_precision {
+  MIF_PRECISION_FP64 = 0,  /* plain form: FP64 fields, stencils, transforms  */
+  MIF_PRECISION_MIXED = 1  /* FP32 fields/stencils/transforms, FP64 defect
+                              correction of the pressure solve               */
+} mif_precision;
+
+/* ---- context ------------------------------------------------------------
+ * Replaces Constants::Constants (src/Constants.cpp:58-120) at Py = Pz = 1
+ * together with PressureSolverStructures::PressureSolverStructures
+ * (src/PressureSolverStructures.cpp:13-69): transform plans, eigenvalue
+ * tables and scratch live in the context.  precision is a mif_precision;
+ * device is the HIP device ordinal (-1 = current). */
+int mif_ctx_create(const mif_grid *grid, int precision, int device, mif_ctx **out);
+/* Frees everything the context owns (PressureSolverStructures::
+ * ~PressureSolverStructures, src/PressureSolverStructures.cpp:72-97).  Fields
+ * must be destroyed first. */
+int mif_ctx_destroy(mif_ctx *ctx);
+/* Waits for all work enqueued on the context's stream. */
+int mif_ctx_synchronize(mif_ctx *ctx);
+/* Message of the last failure on this context (or of the last failed
+ * mif_ctx_create when ctx is NULL). */
+const char *mif_last_error(const mif_ctx *ctx);
+/* Derived constants: dt (src/Constants.cpp:69) and the node extents
+ * Nx, Ny, Nz including periodic ghosts (src/Constants.cpp:80-82). */
+int mif_ctx_info(const mif_ctx *ctx, double *dt, uint64_t node_extents[3]);
+/* Number of FP64 defect-correction sweeps of the mixed form's pressure solve
+ * (default 1; 0..4).  Ignored in the FP64 form. */
+int mif_ctx_set_defect_sweeps(mif_ctx *ctx, int sweeps);
+/* Defect-correction sweep count of the context. */
+int mif_ctx_get_defect_sweeps(const mif_ctx *ctx, int *sweeps);
+
+/* ---- fields ---------------------------------------------------------------
+ * Replaces StaggeredTensor(const Constants&, StaggeringDirection)
+ * (include/StaggeredTensor.h:32, src/StaggeredTensor.cpp:5-9); staggering is
+ * a mif_staggering.  The field is zero-initialised. */
+int mif_field_create(mif_ctx *ctx, int staggering, mif_field **out);
+int mif_field_destroy(mif_field *f);
+/* Tensor::sizes() (include/Tensor.h:151-153). */
+int mif_field_extents(const mif_field *f, uint64_t extents[3]);
+/* Copy a host array in the reference layout to the device (Tensor raw_data
+ * writes, include/Tensor.h:115). */
+int mif_field_upload(mif_field *f, const double *host);
+/* Copy the field to a host array in the reference layout. */
+int mif_field_download(const mif_field *f, double *host);
+/* Tensor::swap_data (include/Tensor.h:107-109): exchange device storage. */
+int mif_field_swap(mif_field *a, mif_field *b);
+/* StaggeredTensor::set (src/StaggeredTensor.cpp:205-219) with a known
+ * function at time t (component comp of a vector function). */
+int mif_field_set(mif_field *f, const mif_function *fn, int comp, double t, int include_border);
+/* StaggeredTensor::apply_periodic_bc (src/StaggeredTensor.cpp:221-256). */
+int mif_field_apply_periodic_bc(mif_field *f);
+
+/* ---- velocity -------------------------------------------------------------
+ * VelocityTensor::apply_bc (src/VelocityTensor.cpp:36-233) with the exact
+ * velocity at time t (TimeVectorFunction::set_time, src/VectorFunction.cpp:
+ * 47-51).  vel = {u, v, w}. */
+int mif_velocity_apply_bc(mif_field *const vel[3], const mif_function *exact_velocity, double t);
+
+/* ---- timesteps --------------------------------------------------------------
+ * timestep (include/Timestep.h:16-19, src/Timestep.cpp:151-156): one RK3 step
+ * with an incremental projection per stage, homogeneous Neumann/periodic
+ * pressure increment.  vel, buf, buf2 are {u, v, w} triples. */
+int mif_timestep(mif_field *const vel[3], mif_field *const buf[3], mif_field *const buf2[3],
+                 const mif_function *exact_velocity, double t_n, mif_field *pressure,
+                 mif_field *pressure_buffer);
+/* timestep_nhn (include/Timestep.h:23-27, src/Timestep.cpp:143-149): the
+ * same with non-homogeneous Neumann data from the exact pressure gradient
+ * (all-wall domains only). */
+int mif_timestep_nhn(mif_field *const vel[3], mif_field *const buf[3], mif_field *const buf2[3],
+                     const mif_function *exact_velocity, const mif_function *exact_pressure_gradient,
+                     double t_n, mif_field *pressure, mif_field *pressure_buffer);
+/* timestep_velocity (include/TimestepVelocity.h:15-16,
+ * src/TimestepVelocity.cpp:60-90): velocity-only RK3 with forcing; on return
+ * vel and buf have swapped storage as in the reference. */
+int mif_timestep_velocity(mif_field *const vel[3], mif_field *const buf[3], mif_field *const rhs[3],
+                          const mif_function *exact_velocity, const mif_function *forcing, double t_n);
+
+/* ---- pressure -------------------------------------------------------------
+ * solve_pressure_equation_homogeneous_periodic (include/PressureEquation.h:
+ * 10-13, src/PressureEquation.cpp:266-274): pressure <- solution of
+ * L phi = div(vel)/dt, with periodic ghosts. */
+int mif_solve_pressure_homogeneous_periodic(mif_field *pressure, mif_field *const vel[3], double dt);
+/* solve_pressure_equation_non_homogeneous_neumann (include/PressureEquation.h:
+ * 16-21, src/PressureEquation.cpp:276-286); exact_pressure_gradient is
+ * evaluated at time t (VectorFunction after set_time). */
+int mif_solve_pressure_non_homogeneous_neumann(mif_field *pressure, mif_field *const vel[3],
+                                               const mif_function *exact_pressure_gradient, double t,
+                                               double dt);
+/* The spectral solve alone (src/PressureEquation.cpp:65-264 between
+ * copy_from_staggered and copy_to_staggered): the right-hand side is read
+ * from the owner points of `pressure`, the solution replaces it (owner
+ * points and periodic ghosts). */
+int mif_poisson_solve(mif_field *pressure);
+/* adjust_pressure (include/PressureEquation.h:25-26,
+ * src/PressureEquation.cpp:288-343) with the exact pressure at time t. */
+int mif_adjust_pressure(mif_field *pressure, const mif_function *exact_pressure, double t);
+
+/* ---- diagnostics ------------------------------------------------------------
+ * ErrorL1Norm/ErrorL2Norm/ErrorLInfNorm of the velocity (include/Norms.h:
+ * 19-29, src/Norms.cpp:11-85); out = {L1, L2, Linf}. */
+int mif_error_norms_velocity(mif_field *const vel[3], const mif_function *exact_velocity, double t,
+                             double out[3]);
+/* The scalar versions (include/Norms.h:32-34, src/Norms.cpp:88-120). */
+int mif_error_norms_pressure(const mif_field *pressure, const mif_function *exact_pressure, double t,
+                             double out[3]);
+/* Pressure-gradient field on interior points (pressure_gradient_{u,v,w},
+ * include/PressureGradient.h:9-25, as filled in test/full_test.cpp:141-142). */
+int mif_pressure_gradient(const mif_field *pressure, mif_field *const out[3]);
+
+/* ---- measurement ------------------------------------------------------------
+ * Kernel timing with HIP events recorded on the context's stream around
+ * every launch of the kernels of one class (no reference counterpart; used
+ * to report the roofline of a kernel from inside a run). */
+typedef enum mif_kernel_class {
+  MIF_KC_STAGE = 0,     /* fused RK3 stage (three components)              */
+  MIF_KC_BC = 1,        /* boundary shell                                   */
+  MIF_KC_DIV = 2,       /* divergence -> packed right-hand side            */
+  MIF_KC_FFT_X = 3,     /* x forward/inverse transform passes              */
+  MIF_KC_FFT_Y = 4,     /* y forward/inverse transform passes              */
+  MIF_KC_FFT_Z = 5,     /* fused z forward/scale/inverse pass              */
+  MIF_KC_UNPACK = 6,    /* packed solution -> pressure increment field     */
+  MIF_KC_PROJECT = 7,   /* p += phi, u -= dt grad phi                      */
+  MIF_KC_DEFECT = 8,    /* mixed form: FP64 residual / correction          */
+  MIF_KC_COUNT = 9
+} mif_kernel_class;
+int mif_ctx_set_timing(mif_ctx *ctx, int enable);
+/* Total milliseconds and launch count of a class since timing was enabled
+ * (synchronises the stream). */
+int mif_ctx_kernel_time(mif_ctx *ctx, int kernel_class, double *total_ms, uint64_t *launches);
+const char *mif_kernel_class_name(int kernel_class);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIF_GPU_H */

@@ -1,0 +1,901 @@
+// mif_api.hip -- context, fields and the C ABI entry points (include/mif_gpu.h).
+//
+// The solver sequence of every entry point follows the reference function it
+// replaces; the file:line citations are on the declarations in mif_gpu.h and
+// on the steps below.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mif_gpu.h"
+#include "mif_kernels.h"
+
+using namespace mif;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct TimerRec {
+  int cls;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct mif_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mif_grid desc{};
+  DGrid g{};
+  Layout L{};
+  int precision = 0;
+  size_t esize = 8;
+  int sweeps = 1;
+  // spectral solve
+  FftPlan plan[3]{};
+  double* W64[3]{};
+  float* W32[3]{};
+  double* lam[3]{};
+  double inv_norm = 1;
+  long bp0 = 0, bplane = 0, belems = 0;
+  void* B = nullptr;       // packed working buffer (field precision)
+  double* B64 = nullptr;   // mixed form: FP64 right-hand side
+  double* P64 = nullptr;   // mixed form: FP64 accumulated solution
+  // transfers and reductions
+  double* staging = nullptr;
+  long staging_elems = 0;
+  Red3* partial = nullptr;
+  Red3* red_out = nullptr;
+  int nred = 1024;
+  std::string err;
+  // kernel timing
+  bool timing = false;
+  std::vector<TimerRec> pending;
+  std::vector<hipEvent_t> pool;
+  double total_ms[MIF_KC_COUNT]{};
+  uint64_t launches[MIF_KC_COUNT]{};
+};
+
+struct mif_field {
+  mif_ctx* ctx;
+  int stag;
+  void* data;
+};
+
+namespace {
+
+int fail(mif_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c)
+    c->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail((c), e_ == hipErrorOutOfMemory ? MIF_ERR_OUT_OF_MEMORY : MIF_ERR_HIP, "%s: %s", \
+                  #expr, hipGetErrorString(e_));                                                 \
+  } while (0)
+
+#define LAUNCHCHK(c)                                                                                        \
+  do {                                                                                                      \
+    hipError_t e_ = hipGetLastError();                                                                      \
+    if (e_ != hipSuccess) return fail((c), MIF_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+// Event pair around the launches of one kernel class.
+struct KTimer {
+  mif_ctx* c;
+  int cls;
+  hipEvent_t a = nullptr;
+  KTimer(mif_ctx* c_, int cls_) : c(c_), cls(cls_) {
+    if (!c->timing) return;
+    a = get_event();
+    (void)hipEventRecord(a, c->stream);
+  }
+  ~KTimer() {
+    if (!c->timing || !a) return;
+    hipEvent_t b = get_event();
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({cls, a, b});
+  }
+  hipEvent_t get_event() {
+    if (!c->pool.empty()) {
+      hipEvent_t e = c->pool.back();
+      c->pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+};
+
+void drain_timers(mif_ctx* c) {
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->pending) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    c->total_ms[r.cls] += ms;
+    c->launches[r.cls] += 1;
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->pending.clear();
+}
+
+// Factor a transform length into Stockham radices (8, 4, 2 first, then odd
+// primes).
+void factor_radices(int L, FftPlan& P) {
+  P.nst = 0;
+  int n = L;
+  while (n % 8 == 0) {
+    P.radix[P.nst++] = 8;
+    n /= 8;
+  }
+  while (n % 4 == 0) {
+    P.radix[P.nst++] = 4;
+    n /= 4;
+  }
+  while (n % 2 == 0) {
+    P.radix[P.nst++] = 2;
+    n /= 2;
+  }
+  for (int p = 3; n > 1; p += 2)
+    while (n % p == 0) {
+      P.radix[P.nst++] = p;
+      n /= p;
+    }
+  int Ns = 1;
+  for (int s = 0; s < P.nst; s++) {
+    P.Ns[s] = Ns;
+    Ns *= P.radix[s];
+  }
+}
+
+// compute_eigenvalue_neumann / _periodic (src/PressureSolverStructures.cpp:5-11)
+double eig_neumann(size_t index, double delta, size_t N) {
+  return 2.0 * (std::cos(M_PI * index / (N - 1)) - 1.0) / (delta * delta);
+}
+double eig_periodic(size_t index, double delta, size_t N) {
+  return 2.0 * (std::cos(2.0 * M_PI * index / N) - 1.0) / (delta * delta);
+}
+
+template <typename T>
+T* fptr(mif_field* f) {
+  return (T*)f->data;
+}
+
+bool is_fp64(const mif_ctx* c) { return c->precision == MIF_PRECISION_FP64; }
+
+int check_vel(mif_field* const v[3], mif_ctx** out) {
+  if (!v || !v[0] || !v[1] || !v[2]) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null velocity field");
+  mif_ctx* c = v[0]->ctx;
+  for (int q = 0; q < 3; q++) {
+    if (v[q]->ctx != c) return fail(c, MIF_ERR_INVALID_ARGUMENT, "velocity components belong to different contexts");
+    if (v[q]->stag != q) return fail(c, MIF_ERR_INVALID_ARGUMENT, "velocity component %d has staggering %d", q, v[q]->stag);
+  }
+  *out = c;
+  return MIF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The spectral solve on the packed buffer (src/PressureEquation.cpp:65-264):
+// forward x, forward y, fused z (forward, scale, inverse), inverse y,
+// inverse x.
+// ---------------------------------------------------------------------------
+template <typename T>
+int spectral_solve(mif_ctx* c, T* B) {
+  SpectralScale sc;
+  sc.lam[0] = c->lam[0];
+  sc.lam[1] = c->lam[1];
+  sc.lam[2] = c->lam[2];
+  sc.inv_norm = c->inv_norm;
+  {
+    KTimer t(c, MIF_KC_FFT_X);
+    launch_fft_axis<T>(c->g, 0, -1, 0, c->plan[0], sc, B, c->bp0, c->bplane, c->stream);
+  }
+  {
+    KTimer t(c, MIF_KC_FFT_Y);
+    launch_fft_axis<T>(c->g, 1, -1, 0, c->plan[1], sc, B, c->bp0, c->bplane, c->stream);
+  }
+  {
+    KTimer t(c, MIF_KC_FFT_Z);
+    launch_fft_axis<T>(c->g, 2, 0, 1, c->plan[2], sc, B, c->bp0, c->bplane, c->stream);
+  }
+  {
+    KTimer t(c, MIF_KC_FFT_Y);
+    launch_fft_axis<T>(c->g, 1, +1, 0, c->plan[1], sc, B, c->bp0, c->bplane, c->stream);
+  }
+  {
+    KTimer t(c, MIF_KC_FFT_X);
+    launch_fft_axis<T>(c->g, 0, +1, 0, c->plan[0], sc, B, c->bp0, c->bplane, c->stream);
+  }
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+// Right-hand side from the velocity, solve, and unpack into the pressure
+// field (solve_pressure_equation_* , src/PressureEquation.cpp:266-286).
+template <typename T>
+int pressure_solve(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
+                   int nhn) {
+  T* B = (T*)c->B;
+  {
+    KTimer t(c, MIF_KC_DIV);
+    launch_div_rhs<T, T>(c->g, c->L, fptr<T>(vel[0]), fptr<T>(vel[1]), fptr<T>(vel[2]), B, c->bp0, c->bplane, dt, gp,
+                         tg, nhn, c->stream);
+  }
+  int rc = spectral_solve<T>(c, B);
+  if (rc) return rc;
+  {
+    KTimer t(c, MIF_KC_UNPACK);
+    launch_unpack<T, T>(c->g, c->L, B, c->bp0, c->bplane, fptr<T>(p), c->stream);
+  }
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int pressure_solve_any(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
+                       int nhn) {
+  if (is_fp64(c)) return pressure_solve<double>(c, p, vel, dt, gp, tg, nhn);
+  return pressure_solve<float>(c, p, vel, dt, gp, tg, nhn);
+}
+
+template <typename T>
+int stage(mif_ctx* c, int st, mif_field* const X[3], mif_field* const A[3], mif_field* const O1[3],
+          mif_field* const O2[3], mif_field* p, double a1, double a2, double b) {
+  StageArgs s{};
+  for (int q = 0; q < 3; q++) {
+    s.X[q] = X[q]->data;
+    s.A[q] = A ? A[q]->data : nullptr;
+    s.O1[q] = O1[q]->data;
+    s.O2[q] = O2 ? O2[q]->data : nullptr;
+  }
+  s.p = p->data;
+  s.a1 = a1;
+  s.a2 = a2;
+  s.b = b;
+  KTimer t(c, MIF_KC_STAGE);
+  launch_stage<T>(st, c->g, c->L, s, c->stream);
+  return MIF_OK;
+}
+
+template <typename T>
+int apply_bc(mif_ctx* c, mif_field* const v[3], const FnDesc& f, double t) {
+  void* F[3] = {v[0]->data, v[1]->data, v[2]->data};
+  KTimer tm(c, MIF_KC_BC);
+  launch_bc<T>(c->g, c->L, F, f, t, c->stream);
+  return MIF_OK;
+}
+
+template <typename T>
+int project(mif_ctx* c, mif_field* const v[3], mif_field* p, mif_field* phi, double dts) {
+  KTimer tm(c, MIF_KC_PROJECT);
+  launch_project<T>(c->g, c->L, fptr<T>(v[0]), fptr<T>(v[1]), fptr<T>(v[2]), fptr<T>(p), fptr<T>(phi), dts,
+                    c->stream);
+  return MIF_OK;
+}
+
+// FULL_TIMESTEP (src/Timestep.cpp:97-141).
+template <typename T>
+int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
+                  const FnDesc& ev, const mif_function* gradp, double t_n, mif_field* p, mif_field* pb) {
+  const double dt = c->g.dt;
+  const double dt_1 = 64.0 / 120.0 * dt;
+  const double time_1 = t_n + dt_1;
+  const double dt_2 = 16.0 / 120.0 * dt;
+  const double time_2 = time_1 + dt_2;
+  const double dt_3 = 40.0 / 120.0 * dt;
+  const double final_time = t_n + dt;
+  const int nhn = gradp != nullptr;
+  // TimeVectorFunction::get_difference_over_time(new_time, prev_time) yields
+  // f(prev_time) - f(new_time) (src/Timestep.cpp:89-95, src/VectorFunction.cpp:53-61)
+  FnDesc gd{};
+  if (nhn) {
+    gd = make_fn(gradp);
+    gd.base = gradp->kind;
+    gd.kind = MIF_FN_TIME_DIFFERENCE;
+  }
+  int rc;
+  // Stage 1 (src/Timestep.cpp:107-117)
+  {
+    const double a1 = 64.0 / 120.0 * dt;
+    if ((rc = stage<T>(c, 1, vel, nullptr, buf, buf2, p, a1, 0.0, a1))) return rc;
+  }
+  apply_bc<T>(c, buf, ev, time_1);
+  gd.t0 = time_1;
+  gd.t1 = t_n;
+  if ((rc = pressure_solve<T>(c, pb, buf, dt_1, gd, 0.0, nhn))) return rc;
+  project<T>(c, buf, p, pb, dt_1);
+  // Stage 2 (src/Timestep.cpp:119-129)
+  {
+    const double a1 = -34.0 / 120.0 * dt, a2 = 50.0 / 120.0 * dt;
+    if ((rc = stage<T>(c, 2, buf, buf2, buf2, vel, p, a1, a2, a1 + a2))) return rc;
+  }
+  apply_bc<T>(c, buf2, ev, time_2);
+  gd.t0 = time_2;
+  gd.t1 = time_1;
+  if ((rc = pressure_solve<T>(c, pb, buf2, dt_2, gd, 0.0, nhn))) return rc;
+  project<T>(c, buf2, p, pb, dt_2);
+  // Stage 3 (src/Timestep.cpp:131-140)
+  {
+    const double a2 = -50.0 / 120.0 * dt, a3 = 90.0 / 120.0 * dt;
+    if ((rc = stage<T>(c, 3, buf2, vel, vel, nullptr, p, 0.0, a3, a2 + a3))) return rc;
+  }
+  apply_bc<T>(c, vel, ev, final_time);
+  gd.t0 = final_time;
+  gd.t1 = time_2;
+  if ((rc = pressure_solve<T>(c, pb, vel, dt_3, gd, 0.0, nhn))) return rc;
+  project<T>(c, vel, p, pb, dt_3);
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+// timestep_velocity (src/TimestepVelocity.cpp:60-90).
+template <typename T>
+int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], mif_field* const rhs[3],
+                      const FnDesc& ev, const FnDesc& fo, double t_n) {
+  const double dt = c->g.dt;
+  const double time_1 = t_n + (8.0 / 15.0) * dt;
+  const double time_2 = t_n + (2.0 / 3.0) * dt;
+  const double final_time = t_n + dt;
+  VelStageArgs s{};
+  s.dt = dt;
+  s.forcing = fo;
+  for (int q = 0; q < 3; q++) {
+    s.V[q] = vel[q]->data;
+    s.R[q] = rhs[q]->data;
+    s.B[q] = buf[q]->data;
+  }
+  for (int q = 0; q < 3; q++) s.X[q] = vel[q]->data;
+  s.t = t_n;
+  {
+    KTimer t(c, MIF_KC_STAGE);
+    launch_vel_stage<T>(1, c->g, c->L, s, c->stream);
+  }
+  apply_bc<T>(c, buf, ev, time_1);
+  for (int q = 0; q < 3; q++) s.X[q] = buf[q]->data;
+  s.t = time_1;
+  {
+    KTimer t(c, MIF_KC_STAGE);
+    launch_vel_stage<T>(2, c->g, c->L, s, c->stream);
+  }
+  apply_bc<T>(c, vel, ev, time_2);
+  for (int q = 0; q < 3; q++) s.X[q] = vel[q]->data;
+  s.t = time_2;
+  {
+    KTimer t(c, MIF_KC_STAGE);
+    launch_vel_stage<T>(3, c->g, c->L, s, c->stream);
+  }
+  apply_bc<T>(c, buf, ev, final_time);
+  // velocity.swap_data(velocity_buffer) (src/TimestepVelocity.cpp:89)
+  for (int q = 0; q < 3; q++) std::swap(vel[q]->data, buf[q]->data);
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int reduce_blocks(long total) {
+  long b = (total + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return (int)b;
+}
+
+// Constants, layout, transform plans and scratch of a new context.
+int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precision, int device) {
+  if (device < 0) (void)hipGetDevice(&device);
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) return fail(nullptr, MIF_ERR_NO_DEVICE, "hipSetDevice(%d) failed", device);
+  c->desc = *grid;
+  c->precision = precision;
+  c->esize = precision == MIF_PRECISION_FP64 ? 8 : 4;
+
+  // Constants (src/Constants.cpp:58-120 at Py = Pz = 1)
+  DGrid& g = c->g;
+  const double size[3] = {grid->x_size, grid->y_size, grid->z_size};
+  const double mn[3] = {grid->min_x, grid->min_y, grid->min_z};
+  g.Re = grid->Re;
+  g.dt = grid->final_time / grid->num_time_steps;
+  for (int a = 0; a < 3; a++) {
+    g.Ng[a] = (int)Ng[a];
+    g.periodic[a] = grid->periodic[a] ? 1 : 0;
+    g.N[a] = g.Ng[a] + g.periodic[a];
+    g.base[a] = g.periodic[a] ? -1 : 0;
+    g.npts[a] = g.Ng[a] - g.periodic[a];
+    g.olo[a] = g.periodic[a];
+    const double h = size[a] / (double)(Ng[a] - 1);
+    g.h[a] = h;
+    g.min[a] = mn[a];
+    g.one_over_2h[a] = 1 / (2 * h);
+    g.one_over_8h[a] = 1 / (8 * h);
+    g.one_over_h2Re[a] = 1 / (grid->Re * h * h);
+    g.h_over_2[a] = h / 2;
+    g.one_over_h[a] = 1 / h;
+  }
+  // common padded layout
+  const int align = 64 / (int)c->esize;
+  const int maxx = g.N[0] + 1, maxy = g.N[1] + 1, maxz = g.N[2] + 1;
+  c->L.PX = (maxx + align - 1) / align * align;
+  c->L.SY = maxy;
+  c->L.SZ = maxz;
+  c->L.plane = (long)c->L.PX * c->L.SY;
+  c->L.elems = c->L.plane * c->L.SZ;
+
+  HIPCHK(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+  // pressure-solver structures (src/PressureSolverStructures.cpp:13-69)
+  c->bp0 = (g.npts[0] + align - 1) / align * align;
+  c->bplane = c->bp0 * g.npts[1];
+  c->belems = c->bplane * g.npts[2];
+  HIPCHK(nullptr, hipMalloc(&c->B, c->belems * c->esize));
+  HIPCHK(nullptr, hipMemsetAsync(c->B, 0, c->belems * c->esize, c->stream));
+  if (precision == MIF_PRECISION_MIXED) {
+    HIPCHK(nullptr, hipMalloc(&c->B64, c->belems * sizeof(double)));
+    HIPCHK(nullptr, hipMalloc(&c->P64, c->belems * sizeof(double)));
+  }
+  double norm = 1.0;
+  for (int a = 0; a < 3; a++) {
+    const int n = g.npts[a];
+    FftPlan& P = c->plan[a];
+    P.n = n;
+    P.periodic = g.periodic[a];
+    P.L = g.periodic[a] ? n : 2 * (n - 1);
+    factor_radices(P.L, P);
+    std::vector<double> w64(2 * (size_t)P.L);
+    std::vector<float> w32(2 * (size_t)P.L);
+    for (int e = 0; e < P.L; e++) {
+      const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)e / (long double)P.L;
+      w64[2 * e] = (double)cosl(ang);
+      w64[2 * e + 1] = (double)-sinl(ang);
+      w32[2 * e] = (float)cosl(ang);
+      w32[2 * e + 1] = (float)-sinl(ang);
+    }
+    HIPCHK(nullptr, hipMalloc(&c->W64[a], w64.size() * sizeof(double)));
+    HIPCHK(nullptr, hipMalloc(&c->W32[a], w32.size() * sizeof(float)));
+    HIPCHK(nullptr, hipMemcpy(c->W64[a], w64.data(), w64.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(c->W32[a], w32.data(), w32.size() * sizeof(float), hipMemcpyHostToDevice));
+    P.W64 = c->W64[a];
+    P.W32 = c->W32[a];
+    std::vector<double> lam(n);
+    for (int k = 0; k < n; k++)
+      lam[k] = g.periodic[a] ? eig_periodic(k, g.h[a], n) : eig_neumann(k, g.h[a], n);
+    HIPCHK(nullptr, hipMalloc(&c->lam[a], n * sizeof(double)));
+    HIPCHK(nullptr, hipMemcpy(c->lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    // normalisation after each inverse axis (src/PressureEquation.cpp:167,200,234)
+    norm *= (double)(Ng[a] - 1) * (g.periodic[a] ? 1.0 : 2.0);
+  }
+  c->inv_norm = 1.0 / norm;
+  // transfers and reductions
+  long mx = 0;
+  for (int s = 0; s < 4; s++) {
+    int e[3];
+    extents(g, s, e);
+    mx = std::max(mx, (long)e[0] * e[1] * e[2]);
+  }
+  c->staging_elems = mx;
+  HIPCHK(nullptr, hipMalloc(&c->staging, mx * sizeof(double)));
+  HIPCHK(nullptr, hipMalloc(&c->partial, c->nred * sizeof(Red3)));
+  HIPCHK(nullptr, hipMalloc(&c->red_out, sizeof(Red3)));
+  HIPCHK(nullptr, hipStreamSynchronize(c->stream));
+  return MIF_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* mif_last_error(const mif_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int mif_ctx_create(const mif_grid* grid, int precision, int device, mif_ctx** out) {
+  if (!grid || !out) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (precision != MIF_PRECISION_FP64 && precision != MIF_PRECISION_MIXED)
+    return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "unknown precision %d", precision);
+  const uint64_t Ng[3] = {grid->Nx_global, grid->Ny_global, grid->Nz_global};
+  for (int a = 0; a < 3; a++) {
+    if (Ng[a] < 2 || Ng[a] > (1u << 20)) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "axis %d: %llu nodes", a, (unsigned long long)Ng[a]);
+    if (grid->periodic[a] && Ng[a] < 3)
+      return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "periodic axis %d needs at least 3 nodes", a);
+  }
+  if (grid->num_time_steps == 0 || !(grid->final_time > 0))
+    return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "final_time and num_time_steps must be positive");
+  if (!(grid->Re > 0)) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "Re must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, MIF_ERR_NO_DEVICE, "no HIP device");
+  mif_ctx* c = new (std::nothrow) mif_ctx();
+  if (!c) return fail(nullptr, MIF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  const int rc = ctx_init(c, grid, Ng, precision, device);
+  if (rc != MIF_OK) {
+    mif_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return MIF_OK;
+}
+
+int mif_ctx_destroy(mif_ctx* c) {
+  if (!c) return MIF_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  (void)hipFree(c->B);
+  (void)hipFree(c->B64);
+  (void)hipFree(c->P64);
+  for (int a = 0; a < 3; a++) {
+    (void)hipFree(c->W64[a]);
+    (void)hipFree(c->W32[a]);
+    (void)hipFree(c->lam[a]);
+  }
+  (void)hipFree(c->staging);
+  (void)hipFree(c->partial);
+  (void)hipFree(c->red_out);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MIF_OK;
+}
+
+int mif_ctx_synchronize(mif_ctx* c) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MIF_OK;
+}
+
+int mif_ctx_info(const mif_ctx* c, double* dt, uint64_t ext[3]) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  if (dt) *dt = c->g.dt;
+  if (ext)
+    for (int a = 0; a < 3; a++) ext[a] = (uint64_t)c->g.N[a];
+  return MIF_OK;
+}
+
+int mif_ctx_set_defect_sweeps(mif_ctx* c, int sweeps) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  if (sweeps < 0 || sweeps > 4) return fail(c, MIF_ERR_INVALID_ARGUMENT, "sweeps must be in 0..4");
+  c->sweeps = sweeps;
+  return MIF_OK;
+}
+
+int mif_ctx_get_defect_sweeps(const mif_ctx* c, int* sweeps) {
+  if (!c || !sweeps) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  *sweeps = c->sweeps;
+  return MIF_OK;
+}
+
+int mif_field_create(mif_ctx* c, int stag, mif_field** out) {
+  if (!c || !out) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  if (stag < 0 || stag > 3) return fail(c, MIF_ERR_INVALID_ARGUMENT, "unknown staggering %d", stag);
+  mif_field* f = new (std::nothrow) mif_field{c, stag, nullptr};
+  if (!f) return fail(c, MIF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  hipError_t e = hipMalloc(&f->data, c->L.elems * c->esize);
+  if (e != hipSuccess) {
+    delete f;
+    return fail(c, MIF_ERR_OUT_OF_MEMORY, "field allocation of %ld bytes failed", (long)(c->L.elems * c->esize));
+  }
+  e = hipMemsetAsync(f->data, 0, c->L.elems * c->esize, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(f->data);
+    delete f;
+    return fail(c, MIF_ERR_HIP, "field clear failed: %s", hipGetErrorString(e));
+  }
+  *out = f;
+  return MIF_OK;
+}
+
+int mif_field_destroy(mif_field* f) {
+  if (!f) return MIF_OK;
+  (void)hipStreamSynchronize(f->ctx->stream);
+  (void)hipFree(f->data);
+  delete f;
+  return MIF_OK;
+}
+
+int mif_field_extents(const mif_field* f, uint64_t ext[3]) {
+  if (!f || !ext) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  int e[3];
+  extents(f->ctx->g, f->stag, e);
+  for (int a = 0; a < 3; a++) ext[a] = (uint64_t)e[a];
+  return MIF_OK;
+}
+
+int mif_field_upload(mif_field* f, const double* host) {
+  if (!f || !host) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  mif_ctx* c = f->ctx;
+  int e[3];
+  extents(c->g, f->stag, e);
+  const long n = (long)e[0] * e[1] * e[2];
+  HIPCHK(c, hipMemcpyAsync(c->staging, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (is_fp64(c))
+    launch_from_dense<double>(c->L, e, c->staging, fptr<double>(f), c->stream);
+  else
+    launch_from_dense<float>(c->L, e, c->staging, fptr<float>(f), c->stream);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MIF_OK;
+}
+
+int mif_field_download(const mif_field* f, double* host) {
+  if (!f || !host) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  mif_ctx* c = f->ctx;
+  int e[3];
+  extents(c->g, f->stag, e);
+  const long n = (long)e[0] * e[1] * e[2];
+  if (is_fp64(c))
+    launch_to_dense<double>(c->L, e, (const double*)f->data, c->staging, c->stream);
+  else
+    launch_to_dense<float>(c->L, e, (const float*)f->data, c->staging, c->stream);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(host, c->staging, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MIF_OK;
+}
+
+int mif_field_swap(mif_field* a, mif_field* b) {
+  if (!a || !b) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null field");
+  if (a->ctx != b->ctx) return fail(a->ctx, MIF_ERR_INVALID_ARGUMENT, "fields belong to different contexts");
+  if (a->stag != b->stag) return fail(a->ctx, MIF_ERR_INVALID_ARGUMENT, "fields have different staggering");
+  std::swap(a->data, b->data);
+  return MIF_OK;
+}
+
+int mif_field_set(mif_field* f, const mif_function* fn, int comp, double t, int include_border) {
+  if (!f || !fn) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  mif_ctx* c = f->ctx;
+  if (comp < 0 || comp > 2) return fail(c, MIF_ERR_INVALID_ARGUMENT, "component %d", comp);
+  const FnDesc d = make_fn(fn);
+  if (is_fp64(c))
+    launch_set<double>(c->g, c->L, fptr<double>(f), f->stag, comp, d, t, include_border, c->stream);
+  else
+    launch_set<float>(c->g, c->L, fptr<float>(f), f->stag, comp, d, t, include_border, c->stream);
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int mif_field_apply_periodic_bc(mif_field* f) {
+  if (!f) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null field");
+  mif_ctx* c = f->ctx;
+  if (is_fp64(c))
+    launch_periodic<double>(c->g, c->L, fptr<double>(f), f->stag, c->stream);
+  else
+    launch_periodic<float>(c->g, c->L, fptr<float>(f), f->stag, c->stream);
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int mif_velocity_apply_bc(mif_field* const vel[3], const mif_function* fn, double t) {
+  mif_ctx* c = nullptr;
+  int rc = check_vel(vel, &c);
+  if (rc) return rc;
+  if (!fn) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null function");
+  const FnDesc d = make_fn(fn);
+  if (is_fp64(c))
+    apply_bc<double>(c, vel, d, t);
+  else
+    apply_bc<float>(c, vel, d, t);
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+static int timestep_common(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
+                           const mif_function* ev, const mif_function* gp, double t_n, mif_field* p, mif_field* pb) {
+  mif_ctx *c = nullptr, *c2 = nullptr, *c3 = nullptr;
+  int rc;
+  if ((rc = check_vel(vel, &c)) || (rc = check_vel(buf, &c2)) || (rc = check_vel(buf2, &c3))) return rc;
+  if (c2 != c || c3 != c || !p || !pb || p->ctx != c || pb->ctx != c || !ev)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "inconsistent timestep arguments");
+  if (p->stag != MIF_STAGGER_NONE || pb->stag != MIF_STAGGER_NONE)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "pressure fields must be unstaggered");
+  if (gp && (c->g.periodic[0] || c->g.periodic[1] || c->g.periodic[2]))
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "non-homogeneous Neumann data needs walls on every axis");
+  const FnDesc d = make_fn(ev);
+  if (is_fp64(c)) return full_timestep<double>(c, vel, buf, buf2, d, gp, t_n, p, pb);
+  return full_timestep<float>(c, vel, buf, buf2, d, gp, t_n, p, pb);
+}
+
+int mif_timestep(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
+                 const mif_function* exact_velocity, double t_n, mif_field* p, mif_field* pb) {
+  return timestep_common(vel, buf, buf2, exact_velocity, nullptr, t_n, p, pb);
+}
+
+int mif_timestep_nhn(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
+                     const mif_function* exact_velocity, const mif_function* exact_grad_p, double t_n, mif_field* p,
+                     mif_field* pb) {
+  if (!exact_grad_p) return fail(vel && vel[0] ? vel[0]->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null gradient");
+  return timestep_common(vel, buf, buf2, exact_velocity, exact_grad_p, t_n, p, pb);
+}
+
+int mif_timestep_velocity(mif_field* const vel[3], mif_field* const buf[3], mif_field* const rhs[3],
+                          const mif_function* ev, const mif_function* fo, double t_n) {
+  mif_ctx *c = nullptr, *c2 = nullptr, *c3 = nullptr;
+  int rc;
+  if ((rc = check_vel(vel, &c)) || (rc = check_vel(buf, &c2)) || (rc = check_vel(rhs, &c3))) return rc;
+  if (c2 != c || c3 != c || !ev || !fo) return fail(c, MIF_ERR_INVALID_ARGUMENT, "inconsistent arguments");
+  const FnDesc d = make_fn(ev), f = make_fn(fo);
+  if (is_fp64(c)) return velocity_timestep<double>(c, vel, buf, rhs, d, f, t_n);
+  return velocity_timestep<float>(c, vel, buf, rhs, d, f, t_n);
+}
+
+int mif_solve_pressure_homogeneous_periodic(mif_field* p, mif_field* const vel[3], double dt) {
+  mif_ctx* c = nullptr;
+  int rc = check_vel(vel, &c);
+  if (rc) return rc;
+  if (!p || p->ctx != c || p->stag != MIF_STAGGER_NONE) return fail(c, MIF_ERR_INVALID_ARGUMENT, "bad pressure field");
+  FnDesc none{};
+  return pressure_solve_any(c, p, vel, dt, none, 0.0, 0);
+}
+
+int mif_solve_pressure_non_homogeneous_neumann(mif_field* p, mif_field* const vel[3], const mif_function* gp,
+                                               double t, double dt) {
+  mif_ctx* c = nullptr;
+  int rc = check_vel(vel, &c);
+  if (rc) return rc;
+  if (!p || p->ctx != c || p->stag != MIF_STAGGER_NONE || !gp)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "bad pressure field or gradient");
+  if (c->g.periodic[0] || c->g.periodic[1] || c->g.periodic[2])
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "non-homogeneous Neumann data needs walls on every axis");
+  const FnDesc d = make_fn(gp);
+  return pressure_solve_any(c, p, vel, dt, d, t, 1);
+}
+
+int mif_poisson_solve(mif_field* p) {
+  if (!p || p->stag != MIF_STAGGER_NONE) return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad field");
+  mif_ctx* c = p->ctx;
+  if (is_fp64(c)) {
+    launch_pack<double, double>(c->g, c->L, fptr<double>(p), (double*)c->B, c->bp0, c->bplane, c->stream);
+    int rc = spectral_solve<double>(c, (double*)c->B);
+    if (rc) return rc;
+    launch_unpack<double, double>(c->g, c->L, (double*)c->B, c->bp0, c->bplane, fptr<double>(p), c->stream);
+  } else {
+    launch_pack<float, float>(c->g, c->L, fptr<float>(p), (float*)c->B, c->bp0, c->bplane, c->stream);
+    int rc = spectral_solve<float>(c, (float*)c->B);
+    if (rc) return rc;
+    launch_unpack<float, float>(c->g, c->L, (float*)c->B, c->bp0, c->bplane, fptr<float>(p), c->stream);
+  }
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int mif_adjust_pressure(mif_field* p, const mif_function* fn, double t) {
+  if (!p || !fn || p->stag != MIF_STAGGER_NONE)
+    return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad arguments");
+  mif_ctx* c = p->ctx;
+  const FnDesc d = make_fn(fn);
+  const long total = (long)c->g.npts[0] * c->g.npts[1] * c->g.npts[2];
+  const int nb = reduce_blocks(total);
+  if (is_fp64(c))
+    launch_norm_pressure<double>(c->g, c->L, fptr<double>(p), d, t, 1, c->partial, nb, c->red_out, c->stream);
+  else
+    launch_norm_pressure<float>(c->g, c->L, fptr<float>(p), d, t, 1, c->partial, nb, c->red_out, c->stream);
+  LAUNCHCHK(c);
+  Red3 r;
+  HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(Red3), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the mean over unique nodes (src/PressureEquation.cpp:312-315), added to all points
+  const double difference = r.s1 / (double)((size_t)c->g.npts[0] * c->g.npts[1] * c->g.npts[2]);
+  if (is_fp64(c))
+    launch_add_const<double>(c->g, c->L, fptr<double>(p), difference, c->stream);
+  else
+    launch_add_const<float>(c->g, c->L, fptr<float>(p), difference, c->stream);
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int mif_error_norms_velocity(mif_field* const vel[3], const mif_function* fn, double t, double out[3]) {
+  mif_ctx* c = nullptr;
+  int rc = check_vel(vel, &c);
+  if (rc) return rc;
+  if (!fn || !out) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  const FnDesc d = make_fn(fn);
+  const long total = (long)(c->g.N[0] - 2) * (c->g.N[1] - 2) * (c->g.N[2] - 2);
+  const int nb = reduce_blocks(total);
+  if (is_fp64(c))
+    launch_norm_velocity<double>(c->g, c->L, fptr<double>(vel[0]), fptr<double>(vel[1]), fptr<double>(vel[2]), d, t,
+                                 c->partial, nb, c->red_out, c->stream);
+  else
+    launch_norm_velocity<float>(c->g, c->L, fptr<float>(vel[0]), fptr<float>(vel[1]), fptr<float>(vel[2]), d, t,
+                                c->partial, nb, c->red_out, c->stream);
+  LAUNCHCHK(c);
+  Red3 r;
+  HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(Red3), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double vol = c->g.h[0] * c->g.h[1] * c->g.h[2];
+  out[0] = r.s1 * c->g.h[0] * c->g.h[1] * c->g.h[2];
+  out[1] = std::sqrt(r.s2 * vol);
+  out[2] = r.mx;
+  if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2]))
+    return fail(c, MIF_ERR_NON_FINITE, "non-finite velocity error norm");
+  return MIF_OK;
+}
+
+int mif_error_norms_pressure(const mif_field* p, const mif_function* fn, double t, double out[3]) {
+  if (!p || !fn || !out || p->stag != MIF_STAGGER_NONE)
+    return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad arguments");
+  mif_ctx* c = p->ctx;
+  const FnDesc d = make_fn(fn);
+  const long total = (long)c->g.npts[0] * c->g.npts[1] * c->g.npts[2];
+  const int nb = reduce_blocks(total);
+  if (is_fp64(c))
+    launch_norm_pressure<double>(c->g, c->L, (const double*)p->data, d, t, 0, c->partial, nb, c->red_out, c->stream);
+  else
+    launch_norm_pressure<float>(c->g, c->L, (const float*)p->data, d, t, 0, c->partial, nb, c->red_out, c->stream);
+  LAUNCHCHK(c);
+  Red3 r;
+  HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(Red3), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double vol = c->g.h[0] * c->g.h[1] * c->g.h[2];
+  out[0] = r.s1 * c->g.h[0] * c->g.h[1] * c->g.h[2];
+  out[1] = std::sqrt(r.s2 * vol);
+  out[2] = r.mx;
+  if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2]))
+    return fail(c, MIF_ERR_NON_FINITE, "non-finite pressure error norm");
+  return MIF_OK;
+}
+
+int mif_pressure_gradient(const mif_field* p, mif_field* const out[3]) {
+  mif_ctx* c = nullptr;
+  int rc = check_vel(out, &c);
+  if (rc) return rc;
+  if (!p || p->ctx != c || p->stag != MIF_STAGGER_NONE) return fail(c, MIF_ERR_INVALID_ARGUMENT, "bad pressure");
+  if (is_fp64(c))
+    launch_pgrad<double>(c->g, c->L, (const double*)p->data, fptr<double>(out[0]), fptr<double>(out[1]),
+                         fptr<double>(out[2]), c->stream);
+  else
+    launch_pgrad<float>(c->g, c->L, (const float*)p->data, fptr<float>(out[0]), fptr<float>(out[1]),
+                        fptr<float>(out[2]), c->stream);
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int mif_ctx_set_timing(mif_ctx* c, int enable) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  drain_timers(c);
+  c->timing = enable != 0;
+  for (int q = 0; q < MIF_KC_COUNT; q++) {
+    c->total_ms[q] = 0;
+    c->launches[q] = 0;
+  }
+  return MIF_OK;
+}
+
+int mif_ctx_kernel_time(mif_ctx* c, int cls, double* total_ms, uint64_t* launches) {
+  if (!c || cls < 0 || cls >= MIF_KC_COUNT) return fail(c, MIF_ERR_INVALID_ARGUMENT, "bad kernel class");
+  drain_timers(c);
+  if (total_ms) *total_ms = c->total_ms[cls];
+  if (launches) *launches = c->launches[cls];
+  return MIF_OK;
+}
+
+const char* mif_kernel_class_name(int cls) {
+  static const char* names[MIF_KC_COUNT] = {"stage", "bc", "div_rhs", "fft_x", "fft_y", "fft_z_fused",
+                                            "unpack", "project", "defect"};
+  if (cls < 0 || cls >= MIF_KC_COUNT) return "unknown";
+  return names[cls];
+}
+
+}  // extern "C"

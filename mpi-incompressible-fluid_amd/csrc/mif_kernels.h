@@ -1,0 +1,98 @@
+// mif_kernels.h -- launch interfaces of the HIP kernels (host side).
+#pragma once
+
+#include "mif_device.h"
+
+namespace mif {
+
+// Arguments of the fused three-component RK3 stage kernel of the full step
+// (src/Timestep.cpp:10-54).  X is the stencil source, A the pointwise
+// operand, O1/O2 the outputs; a1, a2, b are the stage coefficients (stage 3
+// passes its a3 in a2).
+struct StageArgs {
+  const void* X[3];
+  const void* A[3];
+  void* O1[3];
+  void* O2[3];
+  const void* p;
+  double a1, a2, b;
+};
+
+// Arguments of the velocity-only stage kernel (src/TimestepVelocity.cpp:20-51).
+struct VelStageArgs {
+  const void* X[3];
+  void* V[3];
+  void* R[3];
+  void* B[3];
+  double t, dt;
+  FnDesc forcing;
+};
+
+struct Red3 {
+  double s1, s2, mx;
+};
+
+// One axis of the spectral Poisson solve: complex transform length L (= n on
+// a periodic axis, 2(n-1) on a Neumann axis), its radix factorisation and
+// the twiddle table W[e] = exp(-2 pi i e / L).
+struct FftPlan {
+  int n, L, periodic;
+  int nst;
+  int radix[24];
+  int Ns[24];
+  const double* W64;
+  const float* W32;
+};
+
+// Scaling applied between the forward and inverse transform of the last
+// axis: 1 / ((lx[i] + ly[j] + lz[k]) * norm), zero for the constant mode.
+struct SpectralScale {
+  const double* lam[3];
+  double inv_norm;
+};
+
+template <typename T>
+void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st);
+template <typename T>
+void launch_vel_stage(int stage, const DGrid& g, const Layout& L, const VelStageArgs& s, hipStream_t st);
+template <typename T>
+void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& f, double t, hipStream_t st);
+template <typename T>
+void launch_periodic(const DGrid& g, const Layout& L, T* F, int stag, hipStream_t st);
+template <typename T>
+void launch_set(const DGrid& g, const Layout& L, T* F, int stag, int comp, const FnDesc& f, double t, int border,
+                hipStream_t st);
+template <typename T, typename S>
+void launch_div_rhs(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, S* B, long bp0, long bplane,
+                    double dt, const FnDesc& gp, double tg, int nhn, hipStream_t st);
+template <typename T, typename S>
+void launch_pack(const DGrid& g, const Layout& L, const T* P, S* B, long bp0, long bplane, hipStream_t st);
+template <typename T, typename S>
+void launch_unpack(const DGrid& g, const Layout& L, const S* B, long bp0, long bplane, T* P, hipStream_t st);
+template <typename T>
+void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st);
+template <typename T>
+void launch_pgrad(const DGrid& g, const Layout& L, const T* p, T* u, T* v, T* w, hipStream_t st);
+template <typename T>
+void launch_from_dense(const Layout& L, const int e[3], const double* src, T* dst, hipStream_t st);
+template <typename T>
+void launch_to_dense(const Layout& L, const int e[3], const T* src, double* dst, hipStream_t st);
+template <typename T>
+void launch_copy(long n, const T* src, T* dst, hipStream_t st);
+template <typename T>
+void launch_add_const(const DGrid& g, const Layout& L, T* P, double c, hipStream_t st);
+template <typename T>
+void launch_norm_velocity(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, const FnDesc& f,
+                          double t, Red3* partial, int nblocks, Red3* out, hipStream_t st);
+template <typename T>
+void launch_norm_pressure(const DGrid& g, const Layout& L, const T* p, const FnDesc& f, double t, int adjust,
+                          Red3* partial, int nblocks, Red3* out, hipStream_t st);
+
+// Spectral solve passes on the packed buffer B (extents npts, row pitch bp0,
+// plane pitch bplane).  dir: -1 forward, +1 inverse; axis 2 with fused = 1
+// runs forward, scale, inverse in one pass.
+template <typename T>
+void launch_fft_axis(const DGrid& g, int axis, int dir, int fused, const FftPlan& plan, const SpectralScale& sc, T* B,
+                     long bp0, long bplane, hipStream_t st);
+
+}  // namespace mif

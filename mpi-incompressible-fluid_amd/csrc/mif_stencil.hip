@@ -1,0 +1,741 @@
+// mif_stencil.hip -- stencil, boundary and reduction kernels of the timestep.
+//
+// Each kernel cites the reference loop it replaces.  All kernels take the
+// padded common layout of mif_device.h; T is the storage/arithmetic type of
+// the fields (double for the plain form, float for the mixed form).
+#include "mif_kernels.h"
+
+namespace mif {
+
+// ---------------------------------------------------------------------------
+// Momentum right-hand side at one point (include/MomentumEquation.h:43-255,
+// the OPT_CPU_CACHE branch).  u, v, w point at element (i,j,k) of each
+// component; dy, dz are the row and plane strides.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T rhs_u(const DGrid& g, const T* __restrict__ u, const T* __restrict__ v,
+                          const T* __restrict__ w, long dy, long dz) {
+  const T u13 = u[-1], u11 = u[-dy], u5 = u[-dz], u14 = u[0], u23 = u[dz], u17 = u[dy], u15 = u[1];
+  const T v13 = v[-1], v16 = v[-1 + dy], v14 = v[0], v17 = v[dy];
+  const T w13 = w[-1], w22 = w[-1 + dz], w14 = w[0], w23 = w[dz];
+  const T conv = -u14 * (u15 - u13) * (T)g.one_over_2h[0] - (v17 + v14 + v16 + v13) * (u17 - u11) * (T)g.one_over_8h[1] -
+                 (w23 + w14 + w22 + w13) * (u23 - u5) * (T)g.one_over_8h[2];
+  const T diff = (u15 - 2 * u14 + u13) * (T)g.one_over_h2Re[0] + (u17 - 2 * u14 + u11) * (T)g.one_over_h2Re[1] +
+                 (u23 - 2 * u14 + u5) * (T)g.one_over_h2Re[2];
+  return conv + diff;
+}
+
+template <typename T>
+__device__ inline T rhs_v(const DGrid& g, const T* __restrict__ u, const T* __restrict__ v,
+                          const T* __restrict__ w, long dy, long dz) {
+  const T u11 = u[-dy], u14 = u[0], u12 = u[1 - dy], u15 = u[1];
+  const T v13 = v[-1], v11 = v[-dy], v5 = v[-dz], v14 = v[0], v23 = v[dz], v17 = v[dy], v15 = v[1];
+  const T w11 = w[-dy], w20 = w[-dy + dz], w14 = w[0], w23 = w[dz];
+  const T conv = -(u15 + u14 + u12 + u11) * (v15 - v13) * (T)g.one_over_8h[0] - v14 * (v17 - v11) * (T)g.one_over_2h[1] -
+                 (w23 + w14 + w20 + w11) * (v23 - v5) * (T)g.one_over_8h[2];
+  const T diff = (v15 - 2 * v14 + v13) * (T)g.one_over_h2Re[0] + (v17 - 2 * v14 + v11) * (T)g.one_over_h2Re[1] +
+                 (v23 - 2 * v14 + v5) * (T)g.one_over_h2Re[2];
+  return conv + diff;
+}
+
+template <typename T>
+__device__ inline T rhs_w(const DGrid& g, const T* __restrict__ u, const T* __restrict__ v,
+                          const T* __restrict__ w, long dy, long dz) {
+  const T u5 = u[-dz], u14 = u[0], u6 = u[1 - dz], u15 = u[1];
+  const T v5 = v[-dz], v14 = v[0], v8 = v[dy - dz], v17 = v[dy];
+  const T w13 = w[-1], w11 = w[-dy], w5 = w[-dz], w14 = w[0], w23 = w[dz], w17 = w[dy], w15 = w[1];
+  const T conv = -(u15 + u14 + u6 + u5) * (w15 - w13) * (T)g.one_over_8h[0] -
+                 (v17 + v14 + v8 + v5) * (w17 - w11) * (T)g.one_over_8h[1] - w14 * (w23 - w5) * (T)g.one_over_2h[2];
+  const T diff = (w15 - 2 * w14 + w13) * (T)g.one_over_h2Re[0] + (w17 - 2 * w14 + w11) * (T)g.one_over_h2Re[1] +
+                 (w23 - 2 * w14 + w5) * (T)g.one_over_h2Re[2];
+  return conv + diff;
+}
+
+template <typename T>
+__device__ inline T comp_rhs(int c, const DGrid& g, const T* u, const T* v, const T* w, long dy, long dz) {
+  return c == 0 ? rhs_u(g, u, v, w, dy, dz) : (c == 1 ? rhs_v(g, u, v, w, dy, dz) : rhs_w(g, u, v, w, dy, dz));
+}
+
+// Interior test of component c (STAGGERED_TENSOR_ITERATE_OVER_ALL_POINTS with
+// include_border = false, include/StaggeredTensorMacros.h:6-36).
+__device__ inline bool interior(const DGrid& g, int c, int i, int j, int k) {
+  int e[3];
+  extents(g, c, e);
+  return i >= 1 && j >= 1 && k >= 1 && i <= e[0] - 2 && j <= e[1] - 2 && k <= e[2] - 2;
+}
+
+// ---------------------------------------------------------------------------
+// RK3 stage kernels of the full step (src/Timestep.cpp:10-54).  All three
+// components are updated in one pass; thread (i,j,k) covers the interior
+// point (i,j,k) of every component whose interior contains it.
+//   stage 1: O1 = X + a1*N(X) - b*Gp,          O2 = N(X)        (X = u^n)
+//   stage 2: O1 = X + a1*A + a2*N(X) - b*Gp,   O2 = a2*N(X)     (A = R1)
+//   stage 3: O1 = X - A + a3*N(X) - b*Gp                        (A = S)
+// O1 aliases A in stages 2 and 3 (same point, same thread).
+// ---------------------------------------------------------------------------
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) k_stage(DGrid g, Layout L, StageArgs s, int imax, int jmax, int kmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > imax || j > jmax || k > kmax) return;
+  const long o = L.idx(i, j, k);
+  const long dy = L.PX, dz = L.plane;
+  const T* X0 = (const T*)s.X[0] + o;
+  const T* X1 = (const T*)s.X[1] + o;
+  const T* X2 = (const T*)s.X[2] + o;
+  const T* p = (const T*)s.p + o;
+  const long doff[3] = {1, dy, dz};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (!interior(g, c, i, j, k)) continue;
+    const T rhs = comp_rhs<T>(c, g, X0, X1, X2, dy, dz);
+    // pressure_gradient_{u,v,w} (include/PressureGradient.h:9-25)
+    const T pg = (p[0] - p[-doff[c]]) * (T)g.one_over_h[c];
+    const T x = ((const T*)s.X[c])[o];
+    if (STAGE == 1) {
+      ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs - (T)s.b * pg;
+      ((T*)s.O2[c])[o] = rhs;
+    } else if (STAGE == 2) {
+      const T rhs_1 = ((const T*)s.A[c])[o];
+      const T rhs_2_scaled = (T)s.a2 * rhs;
+      ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs_1 + rhs_2_scaled - (T)s.b * pg;
+      ((T*)s.O2[c])[o] = rhs_2_scaled;
+    } else {
+      const T rhs_2_scaled = -((const T*)s.A[c])[o];
+      ((T*)s.O1[c])[o] = x + rhs_2_scaled + (T)s.a2 * rhs - (T)s.b * pg;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity-only RK3 stages with forcing (src/TimestepVelocity.cpp:20-51).
+//   Y2: R = N(X)+f(t_n);  B = X + dt*a21*R            (X = u, R -> rhs, B -> buf)
+//   Y3: R' = V + dt*(b1*R); V = V + dt*(a31*R + a32*(N(X)+f(t1)))  (X = buf)
+//   U*: B = R + dt*(b3*(N(X)+f(t2)))                  (X = u)
+// ---------------------------------------------------------------------------
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) k_vel_stage(DGrid g, Layout L, VelStageArgs s, int imax, int jmax, int kmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > imax || j > jmax || k > kmax) return;
+  const long o = L.idx(i, j, k);
+  const long dy = L.PX, dz = L.plane;
+  const T* X0 = (const T*)s.X[0] + o;
+  const T* X1 = (const T*)s.X[1] + o;
+  const T* X2 = (const T*)s.X[2] + o;
+  const T dt = (T)s.dt;
+  for (int c = 0; c < 3; c++) {
+    if (!interior(g, c, i, j, k)) continue;
+    // calculate_momentum_rhs_with_forcing_* (include/MomentumEquationForcing.h:10-31)
+    const T f = comp_rhs<T>(c, g, X0, X1, X2, dy, dz) + (T)eval_at(g, s.forcing, c, c, s.t, i, j, k);
+    if (STAGE == 1) {
+      ((T*)s.R[c])[o] = f;
+      ((T*)s.B[c])[o] = ((const T*)s.X[c])[o] + dt * (T)(8.0 / 15.0) * f;
+    } else if (STAGE == 2) {
+      const T r = ((T*)s.R[c])[o];
+      const T vv = ((T*)s.V[c])[o];
+      ((T*)s.R[c])[o] = vv + dt * ((T)(1.0 / 4.0) * r);
+      ((T*)s.V[c])[o] = vv + dt * ((T)(1.0 / 4.0) * r + (T)(5.0 / 12.0) * f);
+    } else {
+      const T r = ((T*)s.R[c])[o];
+      ((T*)s.B[c])[o] = r + dt * ((T)(3.0 / 4.0) * f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions: VelocityTensor::apply_bc (src/VelocityTensor.cpp:
+// 36-233) followed by apply_periodic_bc (src/StaggeredTensor.cpp:221-256),
+// in one pass over the boundary shell of the three components.
+//
+// The reference writes the six Dirichlet faces in the order z-, z+, y-, y+,
+// x-, x+ (so at an edge the later face wins) and then copies ghost planes
+// x, then y, then z (plane 0 <- plane s-2, plane s-1 <- plane 1).  A ghost
+// point therefore ends up holding the value of the point whose periodic
+// coordinates are mapped (0 -> s-2, s-1 -> 1) on every periodic axis; that
+// source point is either interior (untouched by the BC) or on a Dirichlet
+// face (its value is a function evaluation), so every shell point is
+// computed independently and the pass needs no ordering between threads.
+// ---------------------------------------------------------------------------
+__device__ inline double face_value(const DGrid& g, const FnDesc& f, double t, int c, int face, int i, int j, int k) {
+  // faces 0,1 = z-,z+; 2,3 = y-,y+; 4,5 = x-,x+
+  const int axis = 2 - (face >> 1);
+  if (axis != c) return eval_at(g, f, c, c, t, i, j, k);  // tangential: exact value
+  const bool hi = face & 1;
+  if (c == 2) {
+    const int kn = hi ? g.N[2] - 1 : 0;
+    const double wb = eval_at(g, f, 3, 2, t, i, j, kn);
+    const double du_dx = (eval_at(g, f, 0, 0, t, i + 1, j, kn) - eval_at(g, f, 0, 0, t, i, j, kn)) * g.one_over_h[0];
+    const double dv_dy = (eval_at(g, f, 1, 1, t, i, j + 1, kn) - eval_at(g, f, 1, 1, t, i, j, kn)) * g.one_over_h[1];
+    return hi ? wb - g.h_over_2[2] * (du_dx + dv_dy) : wb + g.h_over_2[2] * (du_dx + dv_dy);
+  }
+  if (c == 1) {
+    const int jn = hi ? g.N[1] - 1 : 0;
+    const double vb = eval_at(g, f, 3, 1, t, i, jn, k);
+    const double du_dx = (eval_at(g, f, 0, 0, t, i + 1, jn, k) - eval_at(g, f, 0, 0, t, i, jn, k)) * g.one_over_h[0];
+    const double dw_dz = (eval_at(g, f, 2, 2, t, i, jn, k + 1) - eval_at(g, f, 2, 2, t, i, jn, k)) * g.one_over_h[2];
+    return hi ? vb - g.h_over_2[1] * (du_dx + dw_dz) : vb + g.h_over_2[1] * (du_dx + dw_dz);
+  }
+  // c == 0; the x+ wall node is Nx_domains = Nx_global - 1, derivatives at Nx - 1
+  const int in = hi ? g.N[0] - 1 : 0;
+  const double ub = eval_at(g, f, 3, 0, t, hi ? g.Ng[0] - 1 : 0, j, k);
+  const double dv_dy = (eval_at(g, f, 1, 1, t, in, j + 1, k) - eval_at(g, f, 1, 1, t, in, j, k)) * g.one_over_h[1];
+  const double dw_dz = (eval_at(g, f, 2, 2, t, in, j, k + 1) - eval_at(g, f, 2, 2, t, in, j, k)) * g.one_over_h[2];
+  return hi ? ub - g.h_over_2[0] * (dv_dy + dw_dz) : ub + g.h_over_2[0] * (dv_dy + dw_dz);
+}
+
+struct BCArgs {
+  void* F[3];
+  FnDesc f;
+  double t;
+  int count[3][3];  // number of shell points per (component, face pair)
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
+  const int c = blockIdx.y / 3, pair = blockIdx.y % 3;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.count[c][pair]) return;
+  int e[3];
+  extents(g, c, e);
+  int i, j, k;
+  // decode the shell point: pair 0 = k planes (all i, j), pair 1 = j planes
+  // (all i, inner k), pair 2 = i planes (inner j, inner k)
+  if (pair == 0) {
+    const int area = e[0] * e[1];
+    const int side = q / area, r = q % area;
+    i = r % e[0];
+    j = r / e[0];
+    k = side ? e[2] - 1 : 0;
+  } else if (pair == 1) {
+    const int area = e[0] * (e[2] - 2);
+    const int side = q / area, r = q % area;
+    i = r % e[0];
+    k = r / e[0] + 1;
+    j = side ? e[1] - 1 : 0;
+  } else {
+    const int area = (e[1] - 2) * (e[2] - 2);
+    const int side = q / area, r = q % area;
+    j = r % (e[1] - 2) + 1;
+    k = r / (e[1] - 2) + 1;
+    i = side ? e[0] - 1 : 0;
+  }
+  // periodic ghost source
+  int s[3] = {i, j, k};
+  for (int ax = 0; ax < 3; ax++) {
+    if (!g.periodic[ax]) continue;
+    if (s[ax] == 0)
+      s[ax] = e[ax] - 2;
+    else if (s[ax] == e[ax] - 1)
+      s[ax] = 1;
+  }
+  // last Dirichlet face containing the source point
+  int face = -1;
+  if (!g.periodic[2]) {
+    if (s[2] == 0) face = 0;
+    if (s[2] == e[2] - 1) face = 1;
+  }
+  if (!g.periodic[1]) {
+    if (s[1] == 0) face = 2;
+    if (s[1] == e[1] - 1) face = 3;
+  }
+  if (!g.periodic[0]) {
+    if (s[0] == 0) face = 4;
+    if (s[0] == e[0] - 1) face = 5;
+  }
+  T* F = (T*)a.F[c];
+  T val;
+  if (face >= 0)
+    val = (T)face_value(g, a.f, a.t, c, face, s[0], s[1], s[2]);
+  else
+    val = F[L.idx(s[0], s[1], s[2])];
+  F[L.idx(i, j, k)] = val;
+}
+
+// Periodic ghost copies of one field (StaggeredTensor::apply_periodic_bc,
+// src/StaggeredTensor.cpp:221-256), same composition argument as above.
+template <typename T>
+__global__ void __launch_bounds__(256) k_periodic(DGrid g, Layout L, T* F, int stag, int count0, int count1, int count2) {
+  const int pair = blockIdx.y;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cnt = pair == 0 ? count0 : (pair == 1 ? count1 : count2);
+  if (q >= cnt) return;
+  int e[3];
+  extents(g, stag, e);
+  int i, j, k;
+  if (pair == 0) {
+    const int area = e[0] * e[1];
+    const int side = q / area, r = q % area;
+    i = r % e[0];
+    j = r / e[0];
+    k = side ? e[2] - 1 : 0;
+  } else if (pair == 1) {
+    const int area = e[0] * (e[2] - 2);
+    const int side = q / area, r = q % area;
+    i = r % e[0];
+    k = r / e[0] + 1;
+    j = side ? e[1] - 1 : 0;
+  } else {
+    const int area = (e[1] - 2) * (e[2] - 2);
+    const int side = q / area, r = q % area;
+    j = r % (e[1] - 2) + 1;
+    k = r / (e[1] - 2) + 1;
+    i = side ? e[0] - 1 : 0;
+  }
+  int s[3] = {i, j, k};
+  bool ghost = false;
+  for (int ax = 0; ax < 3; ax++) {
+    if (!g.periodic[ax]) continue;
+    if (s[ax] == 0) {
+      s[ax] = e[ax] - 2;
+      ghost = true;
+    } else if (s[ax] == e[ax] - 1) {
+      s[ax] = 1;
+      ghost = true;
+    }
+  }
+  if (ghost) F[L.idx(i, j, k)] = F[L.idx(s[0], s[1], s[2])];
+}
+
+// ---------------------------------------------------------------------------
+// Field fill from a function (StaggeredTensor::set, src/StaggeredTensor.cpp:
+// 205-219).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_set(DGrid g, Layout L, T* F, int stag, int comp, FnDesc f, double t, int border) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int k = blockIdx.z;
+  int e[3];
+  extents(g, stag, e);
+  if (i >= e[0]) return;
+  const int lo = border ? 0 : 1;
+  if (i < lo || j < lo || k < lo || i > e[0] - 1 - lo || j > e[1] - 1 - lo || k > e[2] - 1 - lo) return;
+  F[L.idx(i, j, k)] = (T)eval_at(g, f, stag, comp, t, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// Pressure right-hand side: div(u)/dt on owner points (compute_rhs_
+// homogeneous_periodic, src/PressureEquation.cpp:59-61, divergence per
+// include/VelocityDivergence.h:9-20), plus the Neumann boundary data of
+// apply_non_homogeneous_neumann (src/PressureEquation.cpp:10-57) when
+// gp.kind != ZERO, written packed into the spectral buffer (owner point
+// (i,j,k) -> (i-olo0, j-olo1, k-olo2), PressureTensor::copy_from_staggered,
+// src/PressureTensor.cpp:16-19).
+// ---------------------------------------------------------------------------
+template <typename T, typename S>
+__global__ void __launch_bounds__(256) k_div_rhs(DGrid g, Layout L, const T* __restrict__ u, const T* __restrict__ v,
+                                                 const T* __restrict__ w, S* __restrict__ B, long bp0, long bplane,
+                                                 double dt, FnDesc gp, double tg, int nhn) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;  // packed owner coordinates
+  const int b = blockIdx.y;
+  const int c = blockIdx.z;
+  if (a >= g.npts[0]) return;
+  const int i = a + g.olo[0], j = b + g.olo[1], k = c + g.olo[2];
+  const long o = L.idx(i, j, k);
+  const double du_dx = ((double)u[o + 1] - (double)u[o]) * g.one_over_h[0];
+  const double dv_dy = ((double)v[o + L.PX] - (double)v[o]) * g.one_over_h[1];
+  const double dw_dz = ((double)w[o + L.plane] - (double)w[o]) * g.one_over_h[2];
+  double r = (du_dx + dv_dy + dw_dz) / dt;
+  if (nhn) {
+    const int sx = g.N[0], sy = g.N[1], sz = g.N[2];
+    if (k == 0) r += eval_at(g, gp, 3, 2, tg, i, j, k) * 2.0 * g.one_over_h[2];
+    if (k == sz - 1) r -= eval_at(g, gp, 3, 2, tg, i, j, k) * 2.0 * g.one_over_h[2];
+    if (j == 0) r += eval_at(g, gp, 3, 1, tg, i, j, k) * 2.0 * g.one_over_h[1];
+    if (j == sy - 1) r -= eval_at(g, gp, 3, 1, tg, i, j, k) * 2.0 * g.one_over_h[1];
+    if (i == 0) r += eval_at(g, gp, 3, 0, tg, i, j, k) * 2.0 * g.one_over_h[0];
+    if (i == sx - 1) r -= eval_at(g, gp, 3, 0, tg, i, j, k) * 2.0 * g.one_over_h[0];
+  }
+  B[a + bp0 * b + bplane * c] = (S)r;
+}
+
+// Copy owner values of a pressure field into the packed buffer (used when
+// the right-hand side is given directly, as in the stand-alone Poisson solve).
+template <typename T, typename S>
+__global__ void __launch_bounds__(256) k_pack(DGrid g, Layout L, const T* __restrict__ P, S* __restrict__ B, long bp0,
+                                              long bplane) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y, c = blockIdx.z;
+  if (a >= g.npts[0]) return;
+  B[a + bp0 * b + bplane * c] = (S)P[L.idx(a + g.olo[0], b + g.olo[1], c + g.olo[2])];
+}
+
+// Unpack the packed solution into every point of a pressure field: owner
+// points and, through the periodic map, the ghost planes
+// (PressureTensor::copy_to_staggered, src/PressureTensor.cpp:21-34).
+template <typename T, typename S>
+__global__ void __launch_bounds__(256) k_unpack(DGrid g, Layout L, const S* __restrict__ B, long bp0, long bplane,
+                                                T* __restrict__ P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.N[0]) return;
+  int s[3] = {i, j, k};
+  for (int ax = 0; ax < 3; ax++) {
+    if (!g.periodic[ax]) continue;
+    if (s[ax] == 0) s[ax] = g.N[ax] - 2;
+    else if (s[ax] == g.N[ax] - 1) s[ax] = 1;
+    s[ax] -= 1;
+  }
+  P[L.idx(i, j, k)] = (T)B[s[0] + bp0 * s[1] + bplane * s[2]];
+}
+
+// ---------------------------------------------------------------------------
+// Projection: p += phi on all points (update_pressure, src/Timestep.cpp:
+// 79-81) and u -= dt_s * grad(phi) on interior points
+// (update_velocity_pressure_gradient, src/Timestep.cpp:66-76).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* u, T* v, T* w, T* p, const T* __restrict__ phi,
+                                                 double dts, int imax, int jmax, int kmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > imax || j > jmax || k > kmax) return;
+  const long o = L.idx(i, j, k);
+  if (i < g.N[0] && j < g.N[1] && k < g.N[2]) p[o] += phi[o];
+  const long doff[3] = {1, L.PX, L.plane};
+  T* F[3] = {u, v, w};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (!interior(g, c, i, j, k)) continue;
+    F[c][o] -= (phi[o] - phi[o - doff[c]]) * (T)g.one_over_h[c] * (T)dts;
+  }
+}
+
+// Pressure-gradient field on interior points (test/full_test.cpp:141-142).
+template <typename T>
+__global__ void __launch_bounds__(256) k_pgrad(DGrid g, Layout L, const T* __restrict__ p, T* u, T* v, T* w, int imax,
+                                               int jmax, int kmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > imax || j > jmax || k > kmax) return;
+  const long o = L.idx(i, j, k);
+  const long doff[3] = {1, L.PX, L.plane};
+  T* F[3] = {u, v, w};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (!interior(g, c, i, j, k)) continue;
+    F[c][o] = (p[o] - p[o - doff[c]]) * (T)g.one_over_h[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-layout transfers: dense reference layout (x fastest, exact extents,
+// FP64) <-> padded device layout of type T.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_from_dense(Layout L, int e0, int e1, const double* __restrict__ src, T* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= e0) return;
+  dst[L.idx(i, j, k)] = (T)src[i + (long)e0 * (j + (long)e1 * k)];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_to_dense(Layout L, int e0, int e1, const T* __restrict__ src, double* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= e0) return;
+  dst[i + (long)e0 * (j + (long)e1 * k)] = (double)src[L.idx(i, j, k)];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy(long n, const T* __restrict__ src, T* __restrict__ dst) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = src[q];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_add_const(DGrid g, Layout L, T* P, double c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.N[0]) return;
+  P[L.idx(i, j, k)] += (T)c;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions: wave64 shuffles, then LDS across the block, then one partial
+// per block; a single block folds the partials in a fixed order so results
+// are deterministic.
+// ---------------------------------------------------------------------------
+__device__ inline Red3 red_combine(Red3 a, Red3 b) { return {a.s1 + b.s1, a.s2 + b.s2, fmax(a.mx, b.mx)}; }
+
+__device__ inline Red3 block_reduce(Red3 v) {
+  __shared__ Red3 sh[16];
+  for (int off = 32; off > 0; off >>= 1) {
+    Red3 o;
+    o.s1 = __shfl_down(v.s1, off, 64);
+    o.s2 = __shfl_down(v.s2, off, 64);
+    o.mx = __shfl_down(v.mx, off, 64);
+    v = red_combine(v, o);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  Red3 r{0, 0, 0};
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int q = 0; q < nw; q++) r = red_combine(r, sh[q]);
+  }
+  return r;
+}
+
+// ErrorL1/L2/LInfNorm of the velocity (src/Norms.cpp:11-85): errors after
+// interpolating each component to the interior nodes 1..N-2.
+template <typename T>
+__global__ void __launch_bounds__(256) k_norm_velocity(DGrid g, Layout L, const T* __restrict__ u,
+                                                       const T* __restrict__ v, const T* __restrict__ w, FnDesc f,
+                                                       double t, Red3* partial) {
+  const long n0 = g.N[0] - 2, n1 = g.N[1] - 2, n2 = g.N[2] - 2;
+  const long total = n0 * n1 * n2;
+  Red3 acc{0, 0, 0};
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % n0) + 1, j = (int)((q / n0) % n1) + 1, k = (int)(q / (n0 * n1)) + 1;
+    const double x = g.min[0] + (double)(g.base[0] + i) * g.h[0];
+    const double y = g.min[1] + (double)(g.base[1] + j) * g.h[1];
+    const double z = g.min[2] + (double)(g.base[2] + k) * g.h[2];
+    const long o = L.idx(i, j, k);
+    const double iu = ((double)u[o] + (double)u[o + 1]) / 2.0;
+    const double iv = ((double)v[o] + (double)v[o + L.PX]) / 2.0;
+    const double iw = ((double)w[o] + (double)w[o + L.plane]) / 2.0;
+    const double eu = fn_eval(f, 0, t, x, y, z) - iu;
+    const double ev = fn_eval(f, 1, t, x, y, z) - iv;
+    const double ew = fn_eval(f, 2, t, x, y, z) - iw;
+    const double e2 = eu * eu + ev * ev + ew * ew;
+    acc.s1 += sqrt(e2);
+    acc.s2 += e2;
+    acc.mx = fmax(acc.mx, fmax(fabs(eu), fmax(fabs(ev), fabs(ew))));
+  }
+  const Red3 r = block_reduce(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// Pressure error over owner points (src/Norms.cpp:88-120); with f == NULL
+// kind and mode 1 it accumulates sum(f - p) for adjust_pressure
+// (src/PressureEquation.cpp:288-295).
+template <typename T>
+__global__ void __launch_bounds__(256) k_norm_pressure(DGrid g, Layout L, const T* __restrict__ p, FnDesc f, double t,
+                                                       int adjust, Red3* partial) {
+  const long n0 = g.npts[0], n1 = g.npts[1], n2 = g.npts[2];
+  const long total = n0 * n1 * n2;
+  Red3 acc{0, 0, 0};
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % n0) + g.olo[0], j = (int)((q / n0) % n1) + g.olo[1], k = (int)(q / (n0 * n1)) + g.olo[2];
+    double err;
+    if (adjust) {
+      err = eval_at(g, f, 3, 0, t, i, j, k) - (double)p[L.idx(i, j, k)];
+    } else {
+      const double x = g.min[0] + (double)(g.base[0] + i) * g.h[0];
+      const double y = g.min[1] + (double)(g.base[1] + j) * g.h[1];
+      const double z = g.min[2] + (double)(g.base[2] + k) * g.h[2];
+      err = fn_eval(f, 0, t, x, y, z) - (double)p[L.idx(i, j, k)];
+    }
+    acc.s1 += adjust ? err : fabs(err);
+    acc.s2 += err * err;
+    acc.mx = fmax(acc.mx, fabs(err));
+  }
+  const Red3 r = block_reduce(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(256) k_reduce_final(const Red3* partial, int n, Red3* out) {
+  Red3 acc{0, 0, 0};
+  for (int q = threadIdx.x; q < n; q += blockDim.x) acc = red_combine(acc, partial[q]);
+  const Red3 r = block_reduce(acc);
+  if (threadIdx.x == 0) *out = r;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+template <typename T>
+void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st) {
+  // union of the three components' interiors
+  int eu[3], ev[3], ew[3];
+  extents(g, 0, eu);
+  extents(g, 1, ev);
+  extents(g, 2, ew);
+  const int imax = max(eu[0], max(ev[0], ew[0])) - 2;
+  const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
+  const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
+  dim3 blk(64, 4, 1);
+  dim3 grd((imax + 63) / 64, (jmax + 3) / 4, kmax);
+  if (stage == 1)
+    k_stage<T, 1><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+  else if (stage == 2)
+    k_stage<T, 2><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+  else
+    k_stage<T, 3><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+}
+
+template <typename T>
+void launch_vel_stage(int stage, const DGrid& g, const Layout& L, const VelStageArgs& s, hipStream_t st) {
+  int eu[3], ev[3], ew[3];
+  extents(g, 0, eu);
+  extents(g, 1, ev);
+  extents(g, 2, ew);
+  const int imax = max(eu[0], max(ev[0], ew[0])) - 2;
+  const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
+  const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
+  dim3 blk(64, 4, 1);
+  dim3 grd((imax + 63) / 64, (jmax + 3) / 4, kmax);
+  if (stage == 1)
+    k_vel_stage<T, 1><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+  else if (stage == 2)
+    k_vel_stage<T, 2><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+  else
+    k_vel_stage<T, 3><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+}
+
+static void shell_counts(const DGrid& g, int stag, int cnt[3]) {
+  int e[3];
+  extents(g, stag, e);
+  cnt[0] = 2 * e[0] * e[1];
+  cnt[1] = 2 * e[0] * (e[2] - 2);
+  cnt[2] = 2 * (e[1] - 2) * (e[2] - 2);
+}
+
+template <typename T>
+void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& f, double t, hipStream_t st) {
+  BCArgs a{};
+  int mx = 0;
+  for (int c = 0; c < 3; c++) {
+    a.F[c] = F[c];
+    shell_counts(g, c, a.count[c]);
+    for (int q = 0; q < 3; q++) mx = max(mx, a.count[c][q]);
+  }
+  a.f = f;
+  a.t = t;
+  dim3 grd((mx + 255) / 256, 9, 1);
+  k_bc<T><<<grd, 256, 0, st>>>(g, L, a);
+}
+
+template <typename T>
+void launch_periodic(const DGrid& g, const Layout& L, T* F, int stag, hipStream_t st) {
+  if (!g.periodic[0] && !g.periodic[1] && !g.periodic[2]) return;
+  int cnt[3];
+  shell_counts(g, stag, cnt);
+  const int mx = max(cnt[0], max(cnt[1], cnt[2]));
+  dim3 grd((mx + 255) / 256, 3, 1);
+  k_periodic<T><<<grd, 256, 0, st>>>(g, L, F, stag, cnt[0], cnt[1], cnt[2]);
+}
+
+template <typename T>
+void launch_set(const DGrid& g, const Layout& L, T* F, int stag, int comp, const FnDesc& f, double t, int border,
+                hipStream_t st) {
+  int e[3];
+  extents(g, stag, e);
+  dim3 grd((e[0] + 255) / 256, e[1], e[2]);
+  k_set<T><<<grd, 256, 0, st>>>(g, L, F, stag, comp, f, t, border);
+}
+
+template <typename T, typename S>
+void launch_div_rhs(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, S* B, long bp0, long bplane,
+                    double dt, const FnDesc& gp, double tg, int nhn, hipStream_t st) {
+  dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
+  k_div_rhs<T, S><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg, nhn);
+}
+
+template <typename T, typename S>
+void launch_pack(const DGrid& g, const Layout& L, const T* P, S* B, long bp0, long bplane, hipStream_t st) {
+  dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
+  k_pack<T, S><<<grd, 256, 0, st>>>(g, L, P, B, bp0, bplane);
+}
+
+template <typename T, typename S>
+void launch_unpack(const DGrid& g, const Layout& L, const S* B, long bp0, long bplane, T* P, hipStream_t st) {
+  dim3 grd((g.N[0] + 255) / 256, g.N[1], g.N[2]);
+  k_unpack<T, S><<<grd, 256, 0, st>>>(g, L, B, bp0, bplane, P);
+}
+
+template <typename T>
+void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st) {
+  const int imax = g.N[0];  // covers u's x+ ghost index too (interior test filters)
+  const int jmax = g.N[1];
+  const int kmax = g.N[2];
+  dim3 blk(64, 4, 1);
+  dim3 grd((imax + 1 + 63) / 64, (jmax + 1 + 3) / 4, kmax + 1);
+  k_project<T><<<grd, blk, 0, st>>>(g, L, u, v, w, p, phi, dts, imax, jmax, kmax);
+}
+
+template <typename T>
+void launch_pgrad(const DGrid& g, const Layout& L, const T* p, T* u, T* v, T* w, hipStream_t st) {
+  const int imax = g.N[0], jmax = g.N[1], kmax = g.N[2];
+  dim3 blk(64, 4, 1);
+  dim3 grd((imax + 1 + 63) / 64, (jmax + 1 + 3) / 4, kmax + 1);
+  k_pgrad<T><<<grd, blk, 0, st>>>(g, L, p, u, v, w, imax, jmax, kmax);
+}
+
+template <typename T>
+void launch_from_dense(const Layout& L, const int e[3], const double* src, T* dst, hipStream_t st) {
+  dim3 grd((e[0] + 255) / 256, e[1], e[2]);
+  k_from_dense<T><<<grd, 256, 0, st>>>(L, e[0], e[1], src, dst);
+}
+
+template <typename T>
+void launch_to_dense(const Layout& L, const int e[3], const T* src, double* dst, hipStream_t st) {
+  dim3 grd((e[0] + 255) / 256, e[1], e[2]);
+  k_to_dense<T><<<grd, 256, 0, st>>>(L, e[0], e[1], src, dst);
+}
+
+template <typename T>
+void launch_copy(long n, const T* src, T* dst, hipStream_t st) {
+  k_copy<T><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, src, dst);
+}
+
+template <typename T>
+void launch_add_const(const DGrid& g, const Layout& L, T* P, double c, hipStream_t st) {
+  dim3 grd((g.N[0] + 255) / 256, g.N[1], g.N[2]);
+  k_add_const<T><<<grd, 256, 0, st>>>(g, L, P, c);
+}
+
+template <typename T>
+void launch_norm_velocity(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, const FnDesc& f,
+                          double t, Red3* partial, int nblocks, Red3* out, hipStream_t st) {
+  k_norm_velocity<T><<<nblocks, 256, 0, st>>>(g, L, u, v, w, f, t, partial);
+  k_reduce_final<<<1, 256, 0, st>>>(partial, nblocks, out);
+}
+
+template <typename T>
+void launch_norm_pressure(const DGrid& g, const Layout& L, const T* p, const FnDesc& f, double t, int adjust,
+                          Red3* partial, int nblocks, Red3* out, hipStream_t st) {
+  k_norm_pressure<T><<<nblocks, 256, 0, st>>>(g, L, p, f, t, adjust, partial);
+  k_reduce_final<<<1, 256, 0, st>>>(partial, nblocks, out);
+}
+
+#define MIF_INSTANTIATE(T)                                                                                          \
+  template void launch_stage<T>(int, const DGrid&, const Layout&, const StageArgs&, hipStream_t);                   \
+  template void launch_vel_stage<T>(int, const DGrid&, const Layout&, const VelStageArgs&, hipStream_t);            \
+  template void launch_bc<T>(const DGrid&, const Layout&, void* const*, const FnDesc&, double, hipStream_t);         \
+  template void launch_periodic<T>(const DGrid&, const Layout&, T*, int, hipStream_t);                              \
+  template void launch_set<T>(const DGrid&, const Layout&, T*, int, int, const FnDesc&, double, int, hipStream_t);  \
+  template void launch_project<T>(const DGrid&, const Layout&, T*, T*, T*, T*, const T*, double, hipStream_t);      \
+  template void launch_pgrad<T>(const DGrid&, const Layout&, const T*, T*, T*, T*, hipStream_t);                    \
+  template void launch_from_dense<T>(const Layout&, const int*, const double*, T*, hipStream_t);                    \
+  template void launch_to_dense<T>(const Layout&, const int*, const T*, double*, hipStream_t);                      \
+  template void launch_copy<T>(long, const T*, T*, hipStream_t);                                                    \
+  template void launch_add_const<T>(const DGrid&, const Layout&, T*, double, hipStream_t);                          \
+  template void launch_norm_velocity<T>(const DGrid&, const Layout&, const T*, const T*, const T*, const FnDesc&,   \
+                                        double, Red3*, int, Red3*, hipStream_t);                                    \
+  template void launch_norm_pressure<T>(const DGrid&, const Layout&, const T*, const FnDesc&, double, int, Red3*,   \
+                                        int, Red3*, hipStream_t);
+
+MIF_INSTANTIATE(double)
+MIF_INSTANTIATE(float)
+
+template void launch_div_rhs<double, double>(const DGrid&, const Layout&, const double*, const double*, const double*,
+                                             double*, long, long, double, const FnDesc&, double, int, hipStream_t);
+template void launch_div_rhs<float, float>(const DGrid&, const Layout&, const float*, const float*, const float*,
+                                           float*, long, long, double, const FnDesc&, double, int, hipStream_t);
+template void launch_div_rhs<float, double>(const DGrid&, const Layout&, const float*, const float*, const float*,
+                                            double*, long, long, double, const FnDesc&, double, int, hipStream_t);
+template void launch_pack<double, double>(const DGrid&, const Layout&, const double*, double*, long, long, hipStream_t);
+template void launch_pack<float, float>(const DGrid&, const Layout&, const float*, float*, long, long, hipStream_t);
+template void launch_unpack<double, double>(const DGrid&, const Layout&, const double*, long, long, double*,
+                                            hipStream_t);
+template void launch_unpack<float, float>(const DGrid&, const Layout&, const float*, long, long, float*, hipStream_t);
+
+}  // namespace mif

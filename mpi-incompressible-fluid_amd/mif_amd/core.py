@@ -1,0 +1,292 @@
+"""Python host side over the C ABI (include/mif_gpu.h).
+
+Mirrors the reference's interfaces for the timestep path with the same names
+and argument meaning: Constants (src/Constants.cpp) + PressureSolverStructures
+(src/PressureSolverStructures.cpp) become a Context; StaggeredTensor /
+VelocityTensor become Field / VelocityField; timestep, timestep_nhn,
+timestep_velocity, solve_pressure_equation_*, adjust_pressure and the error
+norms keep their reference names.  Every call goes to the HIP library; a
+failure raises MifError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes
+import sys
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+class MifError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{L.STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _check(rc: int, ctx_ptr=None):
+    if rc != L.OK:
+        msg = L.load().mif_last_error(ctx_ptr)
+        raise MifError(rc, msg.decode() if msg else "")
+
+
+def function(kind: int, Re: float = 1.0, c=(0.0, 0.0, 0.0), base: int = 0, t0: float = 0.0,
+             t1: float = 0.0) -> L.MifFunction:
+    """A known analytic function (the device-side stand-in for the reference's
+    std::function callbacks, include/VectorFunction.h:15-53)."""
+    f = L.MifFunction()
+    f.kind = kind
+    f.base = base
+    f.Re = float(Re)
+    for a in range(3):
+        f.c[a] = float(c[a])
+    f.t0 = float(t0)
+    f.t1 = float(t1)
+    return f
+
+
+@dataclass(frozen=True)
+class GridSpec:
+    """Arguments of the reference's Constants constructor at one rank
+    (include/Constants.h:86-90)."""
+    Nx_global: int
+    Ny_global: int
+    Nz_global: int
+    x_size: float
+    y_size: float
+    z_size: float
+    min_x: float
+    min_y: float
+    min_z: float
+    Re: float
+    final_time: float
+    num_time_steps: int
+    periodic: tuple = (False, False, False)
+
+    def to_c(self) -> L.MifGrid:
+        g = L.MifGrid()
+        g.Nx_global, g.Ny_global, g.Nz_global = self.Nx_global, self.Ny_global, self.Nz_global
+        g.x_size, g.y_size, g.z_size = self.x_size, self.y_size, self.z_size
+        g.min_x, g.min_y, g.min_z = self.min_x, self.min_y, self.min_z
+        g.Re = self.Re
+        g.final_time = self.final_time
+        g.num_time_steps = self.num_time_steps
+        for a in range(3):
+            g.periodic[a] = 1 if self.periodic[a] else 0
+        return g
+
+
+class Context:
+    """One domain on one GPU (Constants + PressureSolverStructures)."""
+
+    def __init__(self, grid: GridSpec, precision: int = L.FP64, device: int = -1):
+        self.lib = L.load()
+        self.grid = grid
+        self.precision = precision
+        ptr = ctypes.c_void_p()
+        _check(self.lib.mif_ctx_create(ctypes.byref(grid.to_c()), precision, device, ctypes.byref(ptr)), None)
+        self.ptr = ptr
+        dt = ctypes.c_double()
+        ext = (ctypes.c_uint64 * 3)()
+        _check(self.lib.mif_ctx_info(self.ptr, ctypes.byref(dt), ext), self.ptr)
+        self.dt = dt.value
+        self.node_extents = tuple(int(e) for e in ext)
+
+    def close(self):
+        if self.ptr:
+            self.lib.mif_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        _check(rc, self.ptr)
+
+    def synchronize(self):
+        self.check(self.lib.mif_ctx_synchronize(self.ptr))
+
+    def set_defect_sweeps(self, n: int):
+        self.check(self.lib.mif_ctx_set_defect_sweeps(self.ptr, n))
+
+    # fields -----------------------------------------------------------------
+    def field(self, staggering: int) -> "Field":
+        return Field(self, staggering)
+
+    def velocity(self) -> "VelocityField":
+        return VelocityField(self)
+
+    def pressure(self) -> "Field":
+        return Field(self, L.STAGGER_NONE)
+
+    # timing -----------------------------------------------------------------
+    def set_timing(self, on: bool):
+        self.check(self.lib.mif_ctx_set_timing(self.ptr, 1 if on else 0))
+
+    def kernel_time(self, cls: int):
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        self.check(self.lib.mif_ctx_kernel_time(self.ptr, cls, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+class Field:
+    """A staggered tensor on the device (StaggeredTensor)."""
+
+    def __init__(self, ctx: Context, staggering: int):
+        self.ctx = ctx
+        self.staggering = staggering
+        ptr = ctypes.c_void_p()
+        ctx.check(ctx.lib.mif_field_create(ctx.ptr, staggering, ctypes.byref(ptr)))
+        self.ptr = ptr
+        ext = (ctypes.c_uint64 * 3)()
+        ctx.check(ctx.lib.mif_field_extents(self.ptr, ext))
+        self.extents = tuple(int(e) for e in ext)
+
+    @property
+    def shape(self):
+        """numpy shape (sz, sy, sx) of the reference layout."""
+        return (self.extents[2], self.extents[1], self.extents[0])
+
+    def close(self):
+        if self.ptr:
+            self.ctx.lib.mif_field_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        self.ctx.check(self.ctx.lib.mif_field_upload(self.ptr, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.mif_field_download(self.ptr, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def set(self, fn: L.MifFunction, comp: int = 0, t: float = 0.0, include_border: bool = True):
+        self.ctx.check(self.ctx.lib.mif_field_set(self.ptr, ctypes.byref(fn), comp, t, 1 if include_border else 0))
+
+    def apply_periodic_bc(self):
+        self.ctx.check(self.ctx.lib.mif_field_apply_periodic_bc(self.ptr))
+
+    def swap_data(self, other: "Field"):
+        self.ctx.check(self.ctx.lib.mif_field_swap(self.ptr, other.ptr))
+
+
+class VelocityField:
+    """u, v, w on their staggered grids (VelocityTensor)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.u = Field(ctx, L.STAGGER_X)
+        self.v = Field(ctx, L.STAGGER_Y)
+        self.w = Field(ctx, L.STAGGER_Z)
+        self.components = (self.u, self.v, self.w)
+
+    def c_array(self):
+        return L._V3(self.u.ptr, self.v.ptr, self.w.ptr)
+
+    def set(self, fn: L.MifFunction, t: float = 0.0, include_border: bool = True):
+        for c, f in enumerate(self.components):
+            f.set(fn, c, t, include_border)
+
+    def upload(self, arrays: Sequence[np.ndarray]):
+        for f, a in zip(self.components, arrays):
+            f.upload(a)
+
+    def download(self):
+        return tuple(f.download() for f in self.components)
+
+    def apply_bc(self, exact_velocity: L.MifFunction, t: float):
+        self.ctx.check(self.ctx.lib.mif_velocity_apply_bc(self.c_array(), ctypes.byref(exact_velocity), t))
+
+    def swap_data(self, other: "VelocityField"):
+        for a, b in zip(self.components, other.components):
+            a.swap_data(b)
+
+
+# ---- reference-named free functions -------------------------------------------
+
+def timestep(velocity: VelocityField, velocity_buffer: VelocityField, velocity_buffer_2: VelocityField,
+             exact_velocity: L.MifFunction, t_n: float, pressure: Field, pressure_buffer: Field):
+    """timestep (include/Timestep.h:16-19)."""
+    ctx = velocity.ctx
+    ctx.check(ctx.lib.mif_timestep(velocity.c_array(), velocity_buffer.c_array(), velocity_buffer_2.c_array(),
+                                   ctypes.byref(exact_velocity), t_n, pressure.ptr, pressure_buffer.ptr))
+
+
+def timestep_nhn(velocity, velocity_buffer, velocity_buffer_2, exact_velocity, exact_pressure_gradient, t_n,
+                 pressure, pressure_buffer):
+    """timestep_nhn (include/Timestep.h:23-27)."""
+    ctx = velocity.ctx
+    ctx.check(ctx.lib.mif_timestep_nhn(velocity.c_array(), velocity_buffer.c_array(), velocity_buffer_2.c_array(),
+                                       ctypes.byref(exact_velocity), ctypes.byref(exact_pressure_gradient), t_n,
+                                       pressure.ptr, pressure_buffer.ptr))
+
+
+def timestep_velocity(velocity, velocity_buffer, rhs_buffer, exact_velocity, forcing, t_n):
+    """timestep_velocity (include/TimestepVelocity.h:15-16); swaps velocity and
+    velocity_buffer storage like the reference."""
+    ctx = velocity.ctx
+    ctx.check(ctx.lib.mif_timestep_velocity(velocity.c_array(), velocity_buffer.c_array(), rhs_buffer.c_array(),
+                                            ctypes.byref(exact_velocity), ctypes.byref(forcing), t_n))
+
+
+def solve_pressure_equation_homogeneous_periodic(pressure: Field, velocity: VelocityField, dt: float):
+    """include/PressureEquation.h:10-13 (the PressureTensor buffer lives in the context)."""
+    ctx = pressure.ctx
+    ctx.check(ctx.lib.mif_solve_pressure_homogeneous_periodic(pressure.ptr, velocity.c_array(), dt))
+
+
+def solve_pressure_equation_non_homogeneous_neumann(pressure, velocity, exact_pressure_gradient, t, dt):
+    """include/PressureEquation.h:16-21; the gradient is evaluated at time t."""
+    ctx = pressure.ctx
+    ctx.check(ctx.lib.mif_solve_pressure_non_homogeneous_neumann(pressure.ptr, velocity.c_array(),
+                                                                ctypes.byref(exact_pressure_gradient), t, dt))
+
+
+def poisson_solve(pressure: Field):
+    """The spectral solve alone on the owner points of `pressure`."""
+    pressure.ctx.check(pressure.ctx.lib.mif_poisson_solve(pressure.ptr))
+
+
+def adjust_pressure(pressure: Field, exact_pressure: L.MifFunction, t: float):
+    """include/PressureEquation.h:25-26."""
+    pressure.ctx.check(pressure.ctx.lib.mif_adjust_pressure(pressure.ptr, ctypes.byref(exact_pressure), t))
+
+
+def error_norms_velocity(velocity: VelocityField, exact_velocity: L.MifFunction, t: float):
+    """(ErrorL1Norm, ErrorL2Norm, ErrorLInfNorm) of the velocity (include/Norms.h:19-29)."""
+    out = (ctypes.c_double * 3)()
+    velocity.ctx.check(velocity.ctx.lib.mif_error_norms_velocity(velocity.c_array(), ctypes.byref(exact_velocity),
+                                                                 t, out))
+    return np.array(list(out))
+
+
+def error_norms_pressure(pressure: Field, exact_pressure: L.MifFunction, t: float):
+    """(ErrorL1Norm, ErrorL2Norm, ErrorLInfNorm) of a scalar field (include/Norms.h:32-34)."""
+    out = (ctypes.c_double * 3)()
+    pressure.ctx.check(pressure.ctx.lib.mif_error_norms_pressure(pressure.ptr, ctypes.byref(exact_pressure), t, out))
+    return np.array(list(out))
+
+
+def pressure_gradient(pressure: Field, out: VelocityField):
+    """Pressure gradient on interior points (include/PressureGradient.h:9-25)."""
+    pressure.ctx.check(pressure.ctx.lib.mif_pressure_gradient(pressure.ptr, out.c_array()))

@@ -1,0 +1,155 @@
+"""Parity of the HIP library (through the C ABI) with the CPU oracle.
+
+Every case runs the same seeded inputs through both and requires the FP64
+results to agree to 1e-11 relative (max-abs difference over max-abs value of
+the oracle field), the tolerance the timestep path is specified to.
+"""
+import numpy as np
+import pytest
+
+import mif_oracle as mo
+from helpers import PI, channel_initial, grids, rel_err, seeded_normal
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+mif = pytest.importorskip("mif_amd")
+
+
+def _print_case(name):
+    print(f"[case] {name}", flush=True)
+
+
+POISSON_CASES = [
+    # (Ng, size, origin, periodic)
+    ((16, 16, 16), (1.0, 1.0, 2.0), (0.0, 0.0, -1.0), (0, 0, 0)),       # all walls, DCT-I L=30
+    ((8, 12, 11), (2 * PI, 2 * PI, 2 * PI), (0.0, 0.0, 0.0), (0, 0, 1)),  # walls x/y, periodic z
+    ((33, 17, 33), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),      # channel
+    ((23, 9, 14), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 1, 1)),          # generic radices (L = 44, 8, 13)
+    ((65, 65, 65), (2 * PI, 2.0, 2 * PI), (0.0, -1.0, 0.0), (1, 0, 1)),  # power-of-two fast path
+]
+
+
+@pytest.mark.parametrize("case", POISSON_CASES, ids=lambda c: "x".join(map(str, c[0])) + "-p" + "".join(map(str, c[3])))
+def test_poisson_solve_matches_oracle(case):
+    Ng, size, origin, per = case
+    _print_case(f"poisson {Ng} periodic={per}")
+    og, gs = grids(Ng, size, origin, 1.0, 1.0, 1, per)
+    o = mo.Oracle(og)
+    b = seeded_normal(tuple(reversed(o.npts)), 20261202)
+    # oracle on the packed array
+    ref = b.copy()
+    o.poisson_packed(ref)
+    # GPU on a pressure field holding b at its owner points
+    ctx = mif.Context(gs)
+    p = ctx.pressure()
+    host = np.zeros(p.shape)
+    lo = [1 if q else 0 for q in per]
+    host[lo[2]:lo[2] + b.shape[0], lo[1]:lo[1] + b.shape[1], lo[0]:lo[0] + b.shape[2]] = b
+    p.upload(host)
+    mif.poisson_solve(p)
+    out = p.download()
+    got = out[lo[2]:lo[2] + b.shape[0], lo[1]:lo[1] + b.shape[1], lo[0]:lo[0] + b.shape[2]]
+    assert rel_err(got, ref) < TOL
+    # ghosts hold the periodic images
+    full = np.zeros(p.shape)
+    full[lo[2]:lo[2] + b.shape[0], lo[1]:lo[1] + b.shape[1], lo[0]:lo[0] + b.shape[2]] = ref
+    o.apply_periodic(full, 3)
+    assert rel_err(out, full) < TOL
+
+
+BC_CASES = [
+    ((12, 10, 9), (1.0, 1.0, 2.0), (0.0, 0.0, -1.0), (0, 0, 0)),
+    ((12, 10, 9), (2 * PI, 2 * PI, 1.0), (0.0, 0.0, 0.0), (1, 1, 0)),
+    ((12, 10, 9), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),
+    ((7, 8, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (1, 1, 1)),
+]
+
+
+@pytest.mark.parametrize("case", BC_CASES, ids=lambda c: "p" + "".join(map(str, c[3])))
+def test_apply_bc_matches_oracle(case):
+    Ng, size, origin, per = case
+    _print_case(f"apply_bc {Ng} periodic={per}")
+    og, gs = grids(Ng, size, origin, 1e3, 1e-4, 1, per)
+    o = mo.Oracle(og)
+    ev = mo.make_fn(mo.FN_ES_VELOCITY, Re=1e3)
+    vel = o.velocity()
+    for c in range(3):
+        vel[c][...] = seeded_normal(vel[c].shape, 7 + c)
+    ctx = mif.Context(gs)
+    gv = ctx.velocity()
+    gv.upload(vel)
+    o.apply_bc(vel, ev, 0.37)
+    gv.apply_bc(mif.function(mif.FN_ES_VELOCITY, Re=1e3), 0.37)
+    for c, got in enumerate(gv.download()):
+        assert rel_err(got, vel[c]) < 1e-14, c
+
+
+def _run_full(Ng, size, origin, Re, T, nsteps, per, init, fn_kind, seed=0):
+    og, gs = grids(Ng, size, origin, Re, T, nsteps, per)
+    o = mo.Oracle(og)
+    ev = mo.make_fn(fn_kind, Re=Re)
+    vel, buf, buf2 = o.velocity(), o.velocity(), o.velocity()
+    p, pb = o.zeros(3), o.zeros(3)
+    if init == "es":
+        o.set_velocity(vel, ev, 0.0)
+        o.set(p, 3, mo.make_fn(mo.FN_ES_PRESSURE, Re=Re), 0, 0.0)
+    else:
+        v0 = channel_initial(o, seed)
+        for c in range(3):
+            vel[c][...] = v0[c]
+    ctx = mif.Context(gs)
+    gvel, gbuf, gbuf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    gp, gpb = ctx.pressure(), ctx.pressure()
+    gvel.upload(vel)
+    gp.upload(p)
+    fn = mif.function(fn_kind, Re=Re)
+    for s in range(nsteps):
+        t = s * o.dt
+        o.timestep(vel, buf, buf2, ev, t, p, pb)
+        mif.timestep(gvel, gbuf, gbuf2, fn, t, gp, gpb)
+    return o, vel, p, gvel.download(), gp.download()
+
+
+def test_full_timestep_ethier_steinman_walls():
+    _print_case("timestep ES 16^3 walls")
+    o, vel, p, gvel, gp = _run_full((16, 16, 16), (1.0, 1.0, 2.0), (0.0, 0.0, -1.0), 1e3, 1e-4, 3, (0, 0, 0), "es",
+                                    mo.FN_ES_VELOCITY)
+    for c in range(3):
+        assert rel_err(gvel[c], vel[c]) < TOL, c
+    assert rel_err(gp, p) < TOL
+
+
+def test_full_timestep_channel_periodic():
+    _print_case("timestep channel 17x9x17")
+    o, vel, p, gvel, gp = _run_full((17, 9, 17), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 5e-3, 4, (1, 0, 1),
+                                    "channel", mo.FN_ZERO, seed=3)
+    for c in range(3):
+        assert rel_err(gvel[c], vel[c]) < TOL, c
+    assert rel_err(gp, p) < TOL
+
+
+def test_timestep_velocity_matches_oracle():
+    _print_case("timestep_velocity 12^3")
+    Re = 1e4
+    og, gs = grids((12, 12, 12), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), Re, 1e-3, 3, (0, 0, 0))
+    o = mo.Oracle(og)
+    ev = mo.make_fn(mo.FN_VT_VELOCITY, Re=Re)
+    fo = mo.make_fn(mo.FN_VT_FORCING, Re=Re)
+    vel, buf, rhs = o.velocity(), o.velocity(), o.velocity()
+    o.set_velocity(vel, ev, 0.0)
+    ctx = mif.Context(gs)
+    gvel, gbuf, grhs = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    gvel.upload(vel)
+    gev, gfo = mif.function(mif.FN_VT_VELOCITY, Re=Re), mif.function(mif.FN_VT_FORCING, Re=Re)
+    for s in range(3):
+        o.timestep_velocity(vel, buf, rhs, ev, fo, s * o.dt)
+        mif.timestep_velocity(gvel, gbuf, grhs, gev, gfo, s * o.dt)
+    for c, got in enumerate(gvel.download()):
+        assert rel_err(got, vel[c]) < TOL, c
+    # norms through the library equal the oracle's on the same field
+    T = 3 * o.dt
+    n_ref = o.error_norms_velocity(vel, ev, T)
+    n_gpu = mif.error_norms_velocity(gvel, gev, T)
+    assert np.allclose(n_gpu, n_ref, rtol=1e-9, atol=0)
