@@ -1,0 +1,151 @@
+"""Pins the oracle's spectral Poisson solve (src/PressureEquation.cpp:65-264)
+by what the mathematics fixes: the exact 7-point operator it inverts, its
+null space, known-answer eigenvector fields and dense solves on tiny grids."""
+import numpy as np
+import pytest
+
+import mif_oracle as mo
+from helpers import PI, grids, seeded_normal
+
+
+def op_matrix_1d(n, h, periodic):
+    """1-D operator the solve diagonalises: circulant [1,-2,1]/h^2 on a
+    periodic axis (n points), node Neumann with ghost reflection (rows
+    (-2,2) and (2,-2)) on a wall axis."""
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = -2
+        if periodic:
+            A[i, (i - 1) % n] += 1
+            A[i, (i + 1) % n] += 1
+        else:
+            if i > 0:
+                A[i, i - 1] += 1
+            if i < n - 1:
+                A[i, i + 1] += 1
+    if not periodic and n > 1:
+        A[0, 1] = 2
+        A[n - 1, n - 2] = 2
+    return A / (h * h)
+
+
+def op_matrix(o):
+    n = o.npts
+    h = list(o.g.h)
+    per = list(o.g.periodic)
+    I = [np.eye(m) for m in n]
+    Ax, Ay, Az = (op_matrix_1d(n[a], h[a], per[a]) for a in range(3))
+    # packed index i + nx*(j + ny*k) -> kron order (k, j, i)
+    return np.kron(I[2], np.kron(I[1], Ax)) + np.kron(I[2], np.kron(Ay, I[0])) + np.kron(Az, np.kron(I[1], I[0]))
+
+
+def weights(o):
+    n = o.npts
+    w = []
+    for a in range(3):
+        v = np.ones(n[a])
+        if not o.g.periodic[a] and n[a] > 1:
+            v[0] = v[-1] = 0.5
+        w.append(v)
+    return np.kron(w[2], np.kron(w[1], w[0]))
+
+
+def apply_op(o, phi):
+    """The 7-point operator on a packed (nz, ny, nx) array."""
+    out = np.zeros_like(phi)
+    h = list(o.g.h)
+    for a, ax in ((0, 2), (1, 1), (2, 0)):
+        n = phi.shape[ax]
+        if o.g.periodic[a]:
+            lap = np.roll(phi, 1, ax) - 2 * phi + np.roll(phi, -1, ax)
+        else:
+            pad_lo = np.take(phi, [1], axis=ax)
+            pad_hi = np.take(phi, [n - 2], axis=ax)
+            ext = np.concatenate([pad_lo, phi, pad_hi], axis=ax)
+            lap = np.take(ext, range(0, n), axis=ax) - 2 * phi + np.take(ext, range(2, n + 2), axis=ax)
+        out += lap / (h[a] * h[a])
+    return out
+
+
+TINY = [
+    ((4, 5, 3), (0, 0, 0)),
+    ((5, 4, 6), (1, 0, 1)),
+    ((6, 5, 5), (0, 1, 1)),
+    ((4, 4, 4), (1, 1, 1)),
+]
+
+
+@pytest.mark.parametrize("Ng,per", TINY)
+def test_matches_dense_constrained_solve(Ng, per):
+    og, _ = grids(Ng, (1.0, 1.3, 0.7), (0.0, 0.0, 0.0), 1.0, 1.0, 1, per)
+    o = mo.Oracle(og)
+    b = seeded_normal(tuple(reversed(o.npts)), 11).ravel()
+    A = op_matrix(o)
+    W = weights(o)
+    one = np.ones_like(b)
+    # the solve discards the constant-mode component of b (left null vector W)
+    bp = b - (W @ b) / (W @ one) * one
+    # and returns the solution whose DCT-I/DFT zero coefficient vanishes,
+    # i.e. W . phi = 0
+    M = np.vstack([A, W[None, :]])
+    rhs = np.concatenate([bp, [0.0]])
+    phi_dense = np.linalg.lstsq(M, rhs, rcond=None)[0]
+    x = b.reshape(tuple(reversed(o.npts))).copy()
+    o.poisson_packed(x)
+    assert np.max(np.abs(x.ravel() - phi_dense)) / np.max(np.abs(phi_dense)) < 1e-11
+
+
+@pytest.mark.parametrize("Ng,per", [((33, 17, 33), (1, 0, 1)), ((16, 16, 16), (0, 0, 0)), ((12, 9, 13), (0, 1, 1))])
+def test_residual_of_exact_operator(Ng, per):
+    og, _ = grids(Ng, (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1.0, 1.0, 1, per)
+    o = mo.Oracle(og)
+    b = seeded_normal(tuple(reversed(o.npts)), 5)
+    W = weights(o).reshape(b.shape)
+    bp = b - (W * b).sum() / W.sum()
+    x = b.copy()
+    o.poisson_packed(x)
+    r = bp - apply_op(o, x)
+    assert np.linalg.norm(r) / np.linalg.norm(bp) < 1e-13
+    # zero weighted mean of the solution
+    assert abs((W * x).sum()) / (np.abs(x).sum()) < 1e-13
+
+
+def test_separable_eigenvector_known_answer():
+    # phi = cos(pi kx i/(nx-1)) * cos(2 pi ky j/ny) * sin(2 pi kz k/nz): an
+    # eigenvector with eigenvalue lx + ly + lz; solving L phi = lambda v
+    # returns v exactly
+    Ng, per = (17, 13, 11), (0, 1, 1)
+    og, _ = grids(Ng, (1.0, 2.0, 3.0), (0.0, 0.0, 0.0), 1.0, 1.0, 1, per)
+    o = mo.Oracle(og)
+    nx, ny, nz = o.npts
+    h = list(o.g.h)
+    i, j, k = np.arange(nx), np.arange(ny), np.arange(nz)
+    kx, ky, kz = 3, 2, 4
+    vx = np.cos(np.pi * kx * i / (nx - 1))
+    vy = np.cos(2 * np.pi * ky * j / ny)
+    vz = np.sin(2 * np.pi * kz * k / nz)
+    lam = (2 * (np.cos(np.pi * kx / (nx - 1)) - 1) / h[0] ** 2 + 2 * (np.cos(2 * np.pi * ky / ny) - 1) / h[1] ** 2 +
+           2 * (np.cos(2 * np.pi * kz / nz) - 1) / h[2] ** 2)
+    v = vz[:, None, None] * vy[None, :, None] * vx[None, None, :]
+    x = lam * v
+    o.poisson_packed(x)
+    assert np.max(np.abs(x - v)) < 1e-12
+
+
+def test_manufactured_pressure_solution_converges():
+    # stand-alone Poisson test of the reference (test/pressure_test_hn.cpp):
+    # velocity -sin x cos y cos z t ..., p = cos x cos y cos z t on
+    # [0, 2pi]^3 with homogeneous Neumann walls, dt = 1; second order in h
+    errs = []
+    for n in (9, 17, 33):
+        og, _ = grids((n, n, n), (2 * PI,) * 3, (0.0, 0.0, 0.0), 1.0, 1.0, 1, (0, 0, 0))
+        o = mo.Oracle(og)
+        vel = o.velocity()
+        o.set_velocity(vel, mo.make_fn(mo.FN_PT_VELOCITY), 1.0)
+        p = o.zeros(3)
+        o.solve_pressure(p, vel, o.dt)
+        pe = mo.make_fn(mo.FN_PT_PRESSURE)
+        o.adjust_pressure(p, pe, 1.0)
+        errs.append(o.error_norms_pressure(p, pe, 1.0)[1])
+    r1, r2 = np.log2(errs[0] / errs[1]), np.log2(errs[1] / errs[2])
+    assert 1.7 < r1 < 2.3 and 1.8 < r2 < 2.2, errs
