@@ -37,6 +37,8 @@ WORDS_PER_LAUNCH = {
     "fft_z_fused": 2.0,
     "unpack": 2.0,
     "project": 9.0,
+    "fft_x_div": 4.0,
+    "fft_x_unpack": 2.0,
 }
 
 

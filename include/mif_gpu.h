@@ -156,7 +156,9 @@ typedef enum mif_kernel_class {
   MIF_KC_UNPACK = 6,    /* packed solution -> pressure increment field     */
   MIF_KC_PROJECT = 7,   /* p += phi, u -= dt grad phi                      */
   MIF_KC_DEFECT = 8,    /* mixed form: FP64 residual / correction          */
-  MIF_KC_COUNT = 9
+  MIF_KC_FFT_X_DIV = 9, /* divergence fused into the x forward pass        */
+  MIF_KC_FFT_X_UNPACK = 10, /* x inverse pass writing the pressure field   */
+  MIF_KC_COUNT = 11
 } mif_kernel_class;
 int mif_ctx_set_timing(mif_ctx *ctx, int enable);
 /* Total milliseconds and launch count of a class since timing was enabled

@@ -203,16 +203,19 @@ int check_vel(mif_field* const v[3], mif_ctx** out) {
 // inverse x.
 // ---------------------------------------------------------------------------
 template <typename T>
-int spectral_solve(mif_ctx* c, T* B) {
+SpectralScale scale_of(const mif_ctx* c) {
   SpectralScale sc;
   sc.lam[0] = c->lam[0];
   sc.lam[1] = c->lam[1];
   sc.lam[2] = c->lam[2];
   sc.inv_norm = c->inv_norm;
-  {
-    KTimer t(c, MIF_KC_FFT_X);
-    launch_fft_axis<T>(c->g, 0, -1, 0, c->plan[0], sc, B, c->bp0, c->bplane, c->stream);
-  }
+  return sc;
+}
+
+// Forward y, fused z (forward, scale, inverse), inverse y.
+template <typename T>
+void spectral_middle(mif_ctx* c, T* B) {
+  const SpectralScale sc = scale_of<T>(c);
   {
     KTimer t(c, MIF_KC_FFT_Y);
     launch_fft_axis<T>(c->g, 1, -1, 0, c->plan[1], sc, B, c->bp0, c->bplane, c->stream);
@@ -225,6 +228,21 @@ int spectral_solve(mif_ctx* c, T* B) {
     KTimer t(c, MIF_KC_FFT_Y);
     launch_fft_axis<T>(c->g, 1, +1, 0, c->plan[1], sc, B, c->bp0, c->bplane, c->stream);
   }
+}
+
+// ---------------------------------------------------------------------------
+// The spectral solve on the packed buffer (src/PressureEquation.cpp:65-264):
+// forward x, forward y, fused z (forward, scale, inverse), inverse y,
+// inverse x.
+// ---------------------------------------------------------------------------
+template <typename T>
+int spectral_solve(mif_ctx* c, T* B) {
+  const SpectralScale sc = scale_of<T>(c);
+  {
+    KTimer t(c, MIF_KC_FFT_X);
+    launch_fft_axis<T>(c->g, 0, -1, 0, c->plan[0], sc, B, c->bp0, c->bplane, c->stream);
+  }
+  spectral_middle<T>(c, B);
   {
     KTimer t(c, MIF_KC_FFT_X);
     launch_fft_axis<T>(c->g, 0, +1, 0, c->plan[0], sc, B, c->bp0, c->bplane, c->stream);
@@ -233,12 +251,31 @@ int spectral_solve(mif_ctx* c, T* B) {
   return MIF_OK;
 }
 
+bool pow2_len(int L) { return L == 64 || L == 128 || L == 256 || L == 512; }
+
 // Right-hand side from the velocity, solve, and unpack into the pressure
-// field (solve_pressure_equation_* , src/PressureEquation.cpp:266-286).
+// field (solve_pressure_equation_* , src/PressureEquation.cpp:266-286).  On
+// power-of-two x lengths the divergence is computed inside the x forward
+// pass and the x inverse pass writes the pressure field directly.
 template <typename T>
 int pressure_solve(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
                    int nhn) {
   T* B = (T*)c->B;
+  const SpectralScale sc = scale_of<T>(c);
+  if (!nhn && pow2_len(c->plan[0].L)) {
+    FusedIO io{vel[0]->data, vel[1]->data, vel[2]->data, p->data, c->L, dt};
+    {
+      KTimer t(c, MIF_KC_FFT_X_DIV);
+      launch_fft_pow2<T>(c->g, 0, 0, c->plan[0], sc, B, c->bp0, c->bplane, &io, c->stream);
+    }
+    spectral_middle<T>(c, B);
+    {
+      KTimer t(c, MIF_KC_FFT_X_UNPACK);
+      launch_fft_pow2<T>(c->g, 0, 1, c->plan[0], sc, B, c->bp0, c->bplane, &io, c->stream);
+    }
+    LAUNCHCHK(c);
+    return MIF_OK;
+  }
   {
     KTimer t(c, MIF_KC_DIV);
     launch_div_rhs<T, T>(c->g, c->L, fptr<T>(vel[0]), fptr<T>(vel[1]), fptr<T>(vel[2]), B, c->bp0, c->bplane, dt, gp,
@@ -892,8 +929,8 @@ int mif_ctx_kernel_time(mif_ctx* c, int cls, double* total_ms, uint64_t* launche
 }
 
 const char* mif_kernel_class_name(int cls) {
-  static const char* names[MIF_KC_COUNT] = {"stage", "bc", "div_rhs", "fft_x", "fft_y", "fft_z_fused",
-                                            "unpack", "project", "defect"};
+  static const char* names[MIF_KC_COUNT] = {"stage",  "bc",      "div_rhs", "fft_x",     "fft_y",       "fft_z_fused",
+                                            "unpack", "project", "defect",  "fft_x_div", "fft_x_unpack"};
   if (cls < 0 || cls >= MIF_KC_COUNT) return "unknown";
   return names[cls];
 }

@@ -102,7 +102,9 @@ __device__ inline double es_dpdx(double t, double x, double y, double z, double 
   return -a * a / 2 * s * exp(-2 * d * d * t / Re);
 }
 
-__device__ inline double fn_base(const FnDesc& f, int kind, int c, double t, double x, double y, double z) {
+// Kept out of line: the closed forms are large and only the boundary,
+// forcing and set-up kernels evaluate them.
+__device__ __noinline__ double fn_base(const FnDesc& f, int kind, int c, double t, double x, double y, double z) {
   switch (kind) {
     case MIF_FN_ZERO:
       return 0.0;

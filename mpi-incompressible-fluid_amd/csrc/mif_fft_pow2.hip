@@ -1,0 +1,419 @@
+// mif_fft_pow2.hip -- transform passes for power-of-two complex lengths.
+//
+// Same passes and slot order as mif_fft.hip (see there for the algorithm
+// and the reference it follows), specialised at compile time for
+// L = 64, 128, 256, 512 (periodic n = L, Neumann n = L/2 + 1), the lengths of
+// the 2^k-interval grids:
+//   * radix-8 Stockham stages (a final radix-4 or -2), all index arithmetic
+//     by shifts, data staged in registers so one LDS buffer suffices;
+//   * twiddles in LDS; one complex pad element every 8 (the radix-8 store
+//     stride) so each group of 8 lanes of a 16-byte LDS access hits
+//     distinct banks;
+//   * global loads go straight into the packed complex lines and the
+//     post-processing writes global memory directly (no staging pass);
+//   * NP line pairs per 256-thread workgroup: 2*NP adjacent x positions on
+//     the strided axes, so every global access is a run of 2*NP*sizeof(T)
+//     contiguous bytes.
+#include "mif_kernels.h"
+
+namespace mif {
+
+namespace p2 {
+
+template <typename T>
+struct Cx {
+  T re, im;
+};
+
+template <typename T>
+__device__ inline Cx<T> add(Cx<T> a, Cx<T> b) {
+  return {a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__device__ inline Cx<T> sub(Cx<T> a, Cx<T> b) {
+  return {a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__device__ inline Cx<T> mul(Cx<T> a, Cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <int SIGN, typename T>
+__device__ inline Cx<T> mul_si(Cx<T> a) {  // a * (SIGN * i)
+  return SIGN < 0 ? Cx<T>{a.im, -a.re} : Cx<T>{-a.im, a.re};
+}
+
+template <int SIGN, int R, typename T>
+__device__ inline void dft(Cx<T>* v) {
+  if constexpr (R == 2) {
+    const Cx<T> a = v[0], b = v[1];
+    v[0] = add(a, b);
+    v[1] = sub(a, b);
+  } else if constexpr (R == 4) {
+    const Cx<T> t0 = add(v[0], v[2]), t1 = sub(v[0], v[2]);
+    const Cx<T> t2 = add(v[1], v[3]), t3 = mul_si<SIGN>(sub(v[1], v[3]));
+    v[0] = add(t0, t2);
+    v[2] = sub(t0, t2);
+    v[1] = add(t1, t3);
+    v[3] = sub(t1, t3);
+  } else {
+    static_assert(R == 8, "radix");
+    Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    Cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    dft<SIGN, 4>(e);
+    dft<SIGN, 4>(o);
+    const T s = (T)0.70710678118654752440;
+    const Cx<T> o1 = mul(o[1], Cx<T>{s, (T)SIGN * s});
+    const Cx<T> o2 = mul_si<SIGN>(o[2]);
+    const Cx<T> o3 = mul(o[3], Cx<T>{-s, (T)SIGN * s});
+    v[0] = add(e[0], o[0]);
+    v[4] = sub(e[0], o[0]);
+    v[1] = add(e[1], o1);
+    v[5] = sub(e[1], o1);
+    v[2] = add(e[2], o2);
+    v[6] = sub(e[2], o2);
+    v[3] = add(e[3], o3);
+    v[7] = sub(e[3], o3);
+  }
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
+// radix of stage s of length L: 8, 8, ..., then 4 or 2
+constexpr int radix_of(int L, int s) {
+  const int m = ilog2(L), full = m / 3;
+  return s < full ? 8 : (1 << (m % 3));
+}
+constexpr int nstages(int L) { return ilog2(L) / 3 + (ilog2(L) % 3 ? 1 : 0); }
+constexpr int ns_of(int L, int s) { return s == 0 ? 1 : ns_of(L, s - 1) * radix_of(L, s - 1); }
+
+// padded LDS index: one pad element every 8
+__device__ constexpr inline int pd(int e) { return e + (e >> 3); }
+
+template <typename T, int L, int NP, int SIGN, int S>
+__device__ inline void stage(Cx<T> (*Z)[L + L / 8], const Cx<T>* tw) {
+  constexpr int R = radix_of(L, S), Ns = ns_of(L, S), m = L / R;
+  constexpr int items = NP * m;
+  constexpr int per = (items + 255) / 256;
+  constexpr int tstep = L / (Ns * R);
+  Cx<T> v[per][R];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int b = 0; b < per; b++) {
+    const int it = t + 256 * b;
+    if (items % 256 == 0 || it < items) {
+      const int p = it / m, jb = it % m, k = jb % Ns;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[b][r] = Z[p][pd(jb + r * m)];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          Cx<T> w = tw[r * k * tstep];
+          if (SIGN > 0) w.im = -w.im;
+          v[b][r] = mul(v[b][r], w);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < per; b++) {
+    const int it = t + 256 * b;
+    if (items % 256 == 0 || it < items) {
+      const int p = it / m, jb = it % m, k = jb % Ns;
+      dft<SIGN, R>(v[b]);
+      const int idx = (jb / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) Z[p][pd(idx + r * Ns)] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int L, int NP, int SIGN, int S = 0>
+__device__ inline void fft(Cx<T> (*Z)[L + L / 8], const Cx<T>* tw) {
+  if constexpr (S < nstages(L)) {
+    stage<T, L, NP, SIGN, S>(Z, tw);
+    fft<T, L, NP, SIGN, S + 1>(Z, tw);
+  }
+}
+
+}  // namespace p2
+
+using p2::Cx;
+using p2::pd;
+
+// MODE 0: forward, 1: inverse, 2: forward + scale + inverse (axis 2 only).
+// FUSED (axis 0 only): MODE 0 reads div(u)/dt from the velocity instead of
+// B (compute_rhs_homogeneous_periodic, src/PressureEquation.cpp:59-61);
+// MODE 1 writes the solution into the pressure field, owner points and
+// periodic ghost images (PressureTensor::copy_to_staggered,
+// src/PressureTensor.cpp:21-34), instead of B.
+template <typename T, int L, int NP, int AXIS, int MODE, bool FUSED>
+__global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W, SpectralScale sc, T* __restrict__ B,
+                                                long bp0, long bplane, int periodic, FusedIO io) {
+  // twiddles live in LDS when the workgroup stays within 64 KB, otherwise
+  // they are read from the (L1-resident) global table
+  constexpr bool TWL = (NP * (L + L / 8) + L) * sizeof(Cx<T>) <= 65536;
+  __shared__ Cx<T> Z[NP][L + L / 8];
+  __shared__ Cx<T> tws[TWL ? L : 1];
+  __shared__ long lbase[2 * NP];
+  const Cx<T>* tw = TWL ? tws : (const Cx<T>*)W;
+  constexpr int NL = 2 * NP;
+  const int t = threadIdx.x;
+  const int n = periodic ? L : L / 2 + 1;
+  const int n0 = g.npts[0], n1 = g.npts[1], n2 = g.npts[2];
+  const long ms = AXIS == 0 ? 1 : (AXIS == 1 ? bp0 : bplane);
+  if (TWL)
+    for (int e = t; e < L; e += 256) tws[e] = Cx<T>{W[2 * e], W[2 * e + 1]};
+  if (t < NL) {
+    long b = -1;
+    if (AXIS == 0) {
+      const long r = (long)blockIdx.x * NL + t;
+      if (r < (long)n1 * n2) b = bp0 * (r % n1) + bplane * (r / n1);
+    } else {
+      const int i = blockIdx.x * NL + t;
+      if (i < n0) b = AXIS == 1 ? i + bplane * blockIdx.y : i + bp0 * blockIdx.y;
+    }
+    lbase[t] = b;
+  }
+  __syncthreads();
+
+  // ---- load: line l -> pair l % NP, real part (l < NP) or imaginary part
+  auto put = [&](int l, int m, T val) {
+    const int p = l % NP;
+    T* z = (T*)&Z[p][pd(m)];
+    z[l < NP ? 0 : 1] = val;
+    if (!periodic && m >= 1 && m <= n - 2) ((T*)&Z[p][pd(L - m)])[l < NP ? 0 : 1] = val;
+  };
+  if (AXIS == 0 && FUSED && MODE == 0) {
+    const T* u = (const T*)io.u;
+    const T* v = (const T*)io.v;
+    const T* w = (const T*)io.w;
+    const long PX = io.L.PX, PL = io.L.plane;
+    for (int l = 0; l < NL; l++) {
+      const long r = (long)blockIdx.x * NL + l;
+      const bool ok = r < (long)n1 * n2;
+      const int j = ok ? (int)(r % n1) + g.olo[1] : 0, k = ok ? (int)(r / n1) + g.olo[2] : 0;
+      for (int m = t; m < n; m += 256) {
+        T val = (T)0;
+        if (ok) {
+          const long o = io.L.idx(m + g.olo[0], j, k);
+          const double du_dx = ((double)u[o + 1] - (double)u[o]) * g.one_over_h[0];
+          const double dv_dy = ((double)v[o + PX] - (double)v[o]) * g.one_over_h[1];
+          const double dw_dz = ((double)w[o + PL] - (double)w[o]) * g.one_over_h[2];
+          val = (T)((du_dx + dv_dy + dw_dz) / io.dt);
+        }
+        put(l, m, val);
+      }
+    }
+  } else if (AXIS == 0) {
+    for (int l = 0; l < NL; l++) {
+      const long b = lbase[l];
+      for (int m = t; m < n; m += 256) put(l, m, b >= 0 ? B[b + m] : (T)0);
+    }
+  } else {
+    const int l = t % NL;
+    const long b = lbase[l];
+    for (int m = t / NL; m < n; m += 256 / NL) put(l, m, b >= 0 ? B[b + m * ms] : (T)0);
+  }
+  __syncthreads();
+
+  if (MODE == 1 && periodic) {
+    // halfcomplex (h1, h2) held as Z[m] = (h1[m], h2[m]) -> Z_k = X1_k + i X2_k
+    for (int it = t; it < NP * (L / 2 + 1); it += 256) {
+      const int p = it / (L / 2 + 1), k = it % (L / 2 + 1);
+      if (k == 0 || k == L / 2) continue;  // Z_0 and Z_{L/2} are already (X1, X2)
+      const Cx<T> a = Z[p][pd(k)], b = Z[p][pd(L - k)];
+      // X1_k = h1[k] + i h1[L-k], X2_k = h2[k] + i h2[L-k]
+      Z[p][pd(k)] = Cx<T>{a.re - b.im, b.re + a.im};
+      Z[p][pd(L - k)] = Cx<T>{a.re + b.im, -b.re + a.im};
+    }
+    __syncthreads();
+  }
+
+  if (MODE == 1)
+    p2::fft<T, L, NP, +1>(Z, tw);
+  else
+    p2::fft<T, L, NP, -1>(Z, tw);
+
+  if (MODE == 2) {
+    // scale every mode by 1/((lx+ly+lz) * norm); constant mode -> 0
+    const int j = blockIdx.y;
+    for (int it = t; it < NP * (L / 2 + 1); it += 256) {
+      const int p = it % NP, k = it / NP;
+      const int i1 = blockIdx.x * NL + p, i2 = i1 + NP;
+      const double lyj = sc.lam[1][j];
+      const double l1 = (i1 < n0 ? sc.lam[0][i1] : 0.0) + lyj, l2 = (i2 < n0 ? sc.lam[0][i2] : 0.0) + lyj;
+      if (periodic) {
+        const double lz = sc.lam[2][k];
+        const T s1 = (i1 == 0 && j == 0 && k == 0) ? (T)0 : (T)((1.0 / (l1 + lz)) * sc.inv_norm);
+        const T s2 = (i2 == 0 && j == 0 && k == 0) ? (T)0 : (T)((1.0 / (l2 + lz)) * sc.inv_norm);
+        const Cx<T> a = Z[p][pd(k)];
+        if (k == 0 || k == L / 2) {
+          Z[p][pd(k)] = Cx<T>{a.re * s1, a.im * s2};
+        } else {
+          const Cx<T> b = Z[p][pd(L - k)];
+          // X1 = (Z_k + conj Z_{L-k})/2, X2 = (Z_k - conj Z_{L-k})/(2i)
+          const T x1r = (a.re + b.re) * (T)0.5 * s1, x1i = (a.im - b.im) * (T)0.5 * s1;
+          const T x2r = (a.im + b.im) * (T)0.5 * s2, x2i = (b.re - a.re) * (T)0.5 * s2;
+          Z[p][pd(k)] = Cx<T>{x1r - x2i, x1i + x2r};
+          Z[p][pd(L - k)] = Cx<T>{x1r + x2i, -x1i + x2r};
+        }
+      } else {
+        // DCT-I coefficients k = 0..n-1 (n = L/2 + 1), even extension kept
+        const double lz = sc.lam[2][k];
+        const T s1 = (i1 == 0 && j == 0 && k == 0) ? (T)0 : (T)((1.0 / (l1 + lz)) * sc.inv_norm);
+        const T s2 = (i2 == 0 && j == 0 && k == 0) ? (T)0 : (T)((1.0 / (l2 + lz)) * sc.inv_norm);
+        const Cx<T> a = Z[p][pd(k)];
+        const Cx<T> sa = Cx<T>{a.re * s1, a.im * s2};
+        Z[p][pd(k)] = sa;
+        if (k >= 1 && k <= L / 2 - 1) Z[p][pd(L - k)] = sa;
+      }
+    }
+    __syncthreads();
+    p2::fft<T, L, NP, +1>(Z, tw);
+  }
+
+  // ---- store
+  auto out_index = [&](int l, int s) -> long { return lbase[l] + (long)s * ms; };
+  if (MODE == 0 && periodic) {
+    // halfcomplex spectra of both lines of each pair
+    auto emit = [&](int p, int k) {
+      const long b1 = lbase[p], b2 = lbase[p + NP];
+      const Cx<T> a = Z[p][pd(k)];
+      if (k == 0 || k == L / 2) {
+        if (b1 >= 0) B[b1 + k * ms] = a.re;
+        if (b2 >= 0) B[b2 + k * ms] = a.im;
+        return;
+      }
+      const Cx<T> b = Z[p][pd(L - k)];
+      if (b1 >= 0) {
+        B[b1 + k * ms] = (a.re + b.re) * (T)0.5;
+        B[b1 + (L - k) * ms] = (a.im - b.im) * (T)0.5;
+      }
+      if (b2 >= 0) {
+        B[b2 + k * ms] = (a.im + b.im) * (T)0.5;
+        B[b2 + (L - k) * ms] = (b.re - a.re) * (T)0.5;
+      }
+    };
+    if (AXIS == 0) {
+      for (int p = 0; p < NP; p++)
+        for (int k = t; k <= L / 2; k += 256) emit(p, k);
+    } else {
+      for (int it = t; it < NP * (L / 2 + 1); it += 256) emit(it % NP, it / NP);
+    }
+  } else {
+    // DCT-I coefficients, or the real lines of an inverse transform
+    if (AXIS == 0 && FUSED && MODE == 1) {
+      T* P = (T*)io.P;
+      const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+      for (int l = 0; l < NL; l++) {
+        const long r = (long)blockIdx.x * NL + l;
+        if (r >= (long)n1 * n2) continue;
+        const int j = (int)(r % n1) + g.olo[1], k = (int)(r / n1) + g.olo[2];
+        // destination rows: the owner row and its periodic images
+        int js[2] = {j, -1}, ks[2] = {k, -1};
+        if (g.periodic[1]) js[1] = j == 1 ? N1 - 1 : (j == N1 - 2 ? 0 : -1);
+        if (g.periodic[2]) ks[1] = k == 1 ? N2 - 1 : (k == N2 - 2 ? 0 : -1);
+        for (int m = t; m < n; m += 256) {
+          const Cx<T> a = Z[l % NP][pd(m)];
+          const T val = l < NP ? a.re : a.im;
+          const int i = m + g.olo[0];
+          const int i2 = g.periodic[0] ? (i == 1 ? N0 - 1 : (i == N0 - 2 ? 0 : -1)) : -1;
+          for (int qk = 0; qk < 2; qk++) {
+            if (ks[qk] < 0) continue;
+            for (int qj = 0; qj < 2; qj++) {
+              if (js[qj] < 0) continue;
+              const long rowb = io.L.idx(0, js[qj], ks[qk]);
+              P[rowb + i] = val;
+              if (i2 >= 0) P[rowb + i2] = val;
+            }
+          }
+        }
+      }
+    } else if (AXIS == 0) {
+      for (int l = 0; l < NL; l++) {
+        const long b = lbase[l];
+        if (b < 0) continue;
+        for (int m = t; m < n; m += 256) {
+          const Cx<T> a = Z[l % NP][pd(m)];
+          B[b + m] = l < NP ? a.re : a.im;
+        }
+      }
+    } else {
+      const int l = t % NL;
+      const long b = lbase[l];
+      if (b >= 0)
+        for (int m = t / NL; m < n; m += 256 / NL) {
+          const Cx<T> a = Z[l % NP][pd(m)];
+          B[out_index(l, m)] = l < NP ? a.re : a.im;
+        }
+    }
+  }
+}
+
+template <typename T, int L, int NP>
+static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
+                      long bplane, const FusedIO* io, hipStream_t st) {
+  FusedIO none{};
+  const T* W;
+  if constexpr (sizeof(T) == 8)
+    W = (const T*)plan.W64;
+  else
+    W = (const T*)plan.W32;
+  dim3 grd;
+  if (axis == 0) {
+    const long lines = (long)g.npts[1] * g.npts[2];
+    grd = dim3((unsigned)((lines + 2 * NP - 1) / (2 * NP)), 1, 1);
+  } else {
+    grd = dim3((unsigned)((g.npts[0] + 2 * NP - 1) / (2 * NP)), axis == 1 ? g.npts[2] : g.npts[1], 1);
+  }
+  const int per = plan.periodic;
+#define MIF_P2(AX, MD) k_fft_p2<T, L, NP, AX, MD, false><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, none)
+  if (axis == 0 && io) {
+    if (mode == 0)
+      k_fft_p2<T, L, NP, 0, 0, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+    else
+      k_fft_p2<T, L, NP, 0, 1, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+  } else if (axis == 0) {
+    if (mode == 0) MIF_P2(0, 0);
+    else MIF_P2(0, 1);
+  } else if (axis == 1) {
+    if (mode == 0) MIF_P2(1, 0);
+    else MIF_P2(1, 1);
+  } else {
+    if (mode == 0) MIF_P2(2, 0);
+    else if (mode == 1) MIF_P2(2, 1);
+    else MIF_P2(2, 2);
+  }
+#undef MIF_P2
+}
+
+// Returns false when the plan's length has no specialised kernel.
+template <typename T>
+bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
+                     long bplane, const FusedIO* io, hipStream_t st) {
+  constexpr bool d = sizeof(T) == 8;
+  switch (plan.L) {
+    case 64:
+      launch_p2<T, 64, 16>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 128:
+      launch_p2<T, 128, 16>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 256:
+      launch_p2<T, 256, (d ? 8 : 16)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 512:
+      launch_p2<T, 512, (d ? 8 : 16)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    default:
+      return false;
+  }
+}
+
+template bool launch_fft_pow2<double>(const DGrid&, int, int, const FftPlan&, const SpectralScale&, double*, long, long,
+                                      const FusedIO*, hipStream_t);
+template bool launch_fft_pow2<float>(const DGrid&, int, int, const FftPlan&, const SpectralScale&, float*, long, long,
+                                     const FusedIO*, hipStream_t);
+
+}  // namespace mif

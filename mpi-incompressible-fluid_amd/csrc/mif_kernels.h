@@ -51,6 +51,17 @@ struct SpectralScale {
   double inv_norm;
 };
 
+// Operands of the x passes fused with the divergence (forward) or with the
+// unpack into the pressure field (inverse).
+struct FusedIO {
+  const void* u;
+  const void* v;
+  const void* w;
+  void* P;
+  Layout L;
+  double dt;
+};
+
 template <typename T>
 void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st);
 template <typename T>
@@ -94,5 +105,11 @@ void launch_norm_pressure(const DGrid& g, const Layout& L, const T* p, const FnD
 template <typename T>
 void launch_fft_axis(const DGrid& g, int axis, int dir, int fused, const FftPlan& plan, const SpectralScale& sc, T* B,
                      long bp0, long bplane, hipStream_t st);
+// Power-of-two x pass fused with the divergence (mode 0) or the unpack into
+// the pressure field (mode 1); returns false when the length has no
+// specialised kernel (the caller then runs the separate kernels).
+template <typename T>
+bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
+                     long bplane, const FusedIO* io, hipStream_t st);
 
 }  // namespace mif

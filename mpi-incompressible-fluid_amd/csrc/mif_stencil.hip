@@ -9,15 +9,15 @@ namespace mif {
 
 // ---------------------------------------------------------------------------
 // Momentum right-hand side at one point (include/MomentumEquation.h:43-255,
-// the OPT_CPU_CACHE branch).  u, v, w point at element (i,j,k) of each
-// component; dy, dz are the row and plane strides.
+// the OPT_CPU_CACHE branch).  X(f, di, dj, dk) returns component f (0 = u,
+// 1 = v, 2 = w) at offset (di, dj, dk) from the point.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ inline T rhs_u(const DGrid& g, const T* __restrict__ u, const T* __restrict__ v,
-                          const T* __restrict__ w, long dy, long dz) {
-  const T u13 = u[-1], u11 = u[-dy], u5 = u[-dz], u14 = u[0], u23 = u[dz], u17 = u[dy], u15 = u[1];
-  const T v13 = v[-1], v16 = v[-1 + dy], v14 = v[0], v17 = v[dy];
-  const T w13 = w[-1], w22 = w[-1 + dz], w14 = w[0], w23 = w[dz];
+template <typename T, typename Tap>
+__device__ inline T rhs_u(const DGrid& g, Tap X) {
+  const T u13 = X(0, -1, 0, 0), u11 = X(0, 0, -1, 0), u5 = X(0, 0, 0, -1), u14 = X(0, 0, 0, 0), u23 = X(0, 0, 0, 1),
+          u17 = X(0, 0, 1, 0), u15 = X(0, 1, 0, 0);
+  const T v13 = X(1, -1, 0, 0), v16 = X(1, -1, 1, 0), v14 = X(1, 0, 0, 0), v17 = X(1, 0, 1, 0);
+  const T w13 = X(2, -1, 0, 0), w22 = X(2, -1, 0, 1), w14 = X(2, 0, 0, 0), w23 = X(2, 0, 0, 1);
   const T conv = -u14 * (u15 - u13) * (T)g.one_over_2h[0] - (v17 + v14 + v16 + v13) * (u17 - u11) * (T)g.one_over_8h[1] -
                  (w23 + w14 + w22 + w13) * (u23 - u5) * (T)g.one_over_8h[2];
   const T diff = (u15 - 2 * u14 + u13) * (T)g.one_over_h2Re[0] + (u17 - 2 * u14 + u11) * (T)g.one_over_h2Re[1] +
@@ -25,12 +25,12 @@ __device__ inline T rhs_u(const DGrid& g, const T* __restrict__ u, const T* __re
   return conv + diff;
 }
 
-template <typename T>
-__device__ inline T rhs_v(const DGrid& g, const T* __restrict__ u, const T* __restrict__ v,
-                          const T* __restrict__ w, long dy, long dz) {
-  const T u11 = u[-dy], u14 = u[0], u12 = u[1 - dy], u15 = u[1];
-  const T v13 = v[-1], v11 = v[-dy], v5 = v[-dz], v14 = v[0], v23 = v[dz], v17 = v[dy], v15 = v[1];
-  const T w11 = w[-dy], w20 = w[-dy + dz], w14 = w[0], w23 = w[dz];
+template <typename T, typename Tap>
+__device__ inline T rhs_v(const DGrid& g, Tap X) {
+  const T u11 = X(0, 0, -1, 0), u14 = X(0, 0, 0, 0), u12 = X(0, 1, -1, 0), u15 = X(0, 1, 0, 0);
+  const T v13 = X(1, -1, 0, 0), v11 = X(1, 0, -1, 0), v5 = X(1, 0, 0, -1), v14 = X(1, 0, 0, 0), v23 = X(1, 0, 0, 1),
+          v17 = X(1, 0, 1, 0), v15 = X(1, 1, 0, 0);
+  const T w11 = X(2, 0, -1, 0), w20 = X(2, 0, -1, 1), w14 = X(2, 0, 0, 0), w23 = X(2, 0, 0, 1);
   const T conv = -(u15 + u14 + u12 + u11) * (v15 - v13) * (T)g.one_over_8h[0] - v14 * (v17 - v11) * (T)g.one_over_2h[1] -
                  (w23 + w14 + w20 + w11) * (v23 - v5) * (T)g.one_over_8h[2];
   const T diff = (v15 - 2 * v14 + v13) * (T)g.one_over_h2Re[0] + (v17 - 2 * v14 + v11) * (T)g.one_over_h2Re[1] +
@@ -38,22 +38,17 @@ __device__ inline T rhs_v(const DGrid& g, const T* __restrict__ u, const T* __re
   return conv + diff;
 }
 
-template <typename T>
-__device__ inline T rhs_w(const DGrid& g, const T* __restrict__ u, const T* __restrict__ v,
-                          const T* __restrict__ w, long dy, long dz) {
-  const T u5 = u[-dz], u14 = u[0], u6 = u[1 - dz], u15 = u[1];
-  const T v5 = v[-dz], v14 = v[0], v8 = v[dy - dz], v17 = v[dy];
-  const T w13 = w[-1], w11 = w[-dy], w5 = w[-dz], w14 = w[0], w23 = w[dz], w17 = w[dy], w15 = w[1];
+template <typename T, typename Tap>
+__device__ inline T rhs_w(const DGrid& g, Tap X) {
+  const T u5 = X(0, 0, 0, -1), u14 = X(0, 0, 0, 0), u6 = X(0, 1, 0, -1), u15 = X(0, 1, 0, 0);
+  const T v5 = X(1, 0, 0, -1), v14 = X(1, 0, 0, 0), v8 = X(1, 0, 1, -1), v17 = X(1, 0, 1, 0);
+  const T w13 = X(2, -1, 0, 0), w11 = X(2, 0, -1, 0), w5 = X(2, 0, 0, -1), w14 = X(2, 0, 0, 0), w23 = X(2, 0, 0, 1),
+          w17 = X(2, 0, 1, 0), w15 = X(2, 1, 0, 0);
   const T conv = -(u15 + u14 + u6 + u5) * (w15 - w13) * (T)g.one_over_8h[0] -
                  (v17 + v14 + v8 + v5) * (w17 - w11) * (T)g.one_over_8h[1] - w14 * (w23 - w5) * (T)g.one_over_2h[2];
   const T diff = (w15 - 2 * w14 + w13) * (T)g.one_over_h2Re[0] + (w17 - 2 * w14 + w11) * (T)g.one_over_h2Re[1] +
                  (w23 - 2 * w14 + w5) * (T)g.one_over_h2Re[2];
   return conv + diff;
-}
-
-template <typename T>
-__device__ inline T comp_rhs(int c, const DGrid& g, const T* u, const T* v, const T* w, long dy, long dz) {
-  return c == 0 ? rhs_u(g, u, v, w, dy, dz) : (c == 1 ? rhs_v(g, u, v, w, dy, dz) : rhs_w(g, u, v, w, dy, dz));
 }
 
 // Interior test of component c (STAGGERED_TENSOR_ITERATE_OVER_ALL_POINTS with
@@ -65,83 +60,103 @@ __device__ inline bool interior(const DGrid& g, int c, int i, int j, int k) {
 }
 
 // ---------------------------------------------------------------------------
-// RK3 stage kernels of the full step (src/Timestep.cpp:10-54).  All three
-// components are updated in one pass; thread (i,j,k) covers the interior
-// point (i,j,k) of every component whose interior contains it.
+// RK3 stage kernels (src/Timestep.cpp:10-54 and src/TimestepVelocity.cpp:
+// 20-51), all three components in one pass.
+//
+// A 64 x 8 workgroup owns an (i, j) tile and marches along k.  The planes
+// k-1, k, k+1 of the stencil source (u, v, w with a one-point halo in i and
+// j) live in an LDS ring buffer, so each source value is fetched from
+// memory about once; every tap of the 15-point stencils is an LDS read.
+//
+// Full step (KIND 1..3):
 //   stage 1: O1 = X + a1*N(X) - b*Gp,          O2 = N(X)        (X = u^n)
 //   stage 2: O1 = X + a1*A + a2*N(X) - b*Gp,   O2 = a2*N(X)     (A = R1)
 //   stage 3: O1 = X - A + a3*N(X) - b*Gp                        (A = S)
-// O1 aliases A in stages 2 and 3 (same point, same thread).
+// Velocity-only step with forcing f (KIND 4..6):
+//   Y2: R = N(X)+f(t_n);  B = X + dt*a21*R                       (X = u)
+//   Y3: R' = V + dt*(b1*R); V = V + dt*(a31*R + a32*(N(X)+f))    (X = buf)
+//   U*: B = R + dt*(b3*(N(X)+f))                                 (X = u)
+// Outputs may alias pointwise operands (same point, same thread); the
+// stencil source is never written.
 // ---------------------------------------------------------------------------
-template <typename T, int STAGE>
-__global__ void __launch_bounds__(256) k_stage(DGrid g, Layout L, StageArgs s, int imax, int jmax, int kmax) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > imax || j > jmax || k > kmax) return;
-  const long o = L.idx(i, j, k);
-  const long dy = L.PX, dz = L.plane;
-  const T* X0 = (const T*)s.X[0] + o;
-  const T* X1 = (const T*)s.X[1] + o;
-  const T* X2 = (const T*)s.X[2] + o;
-  const T* p = (const T*)s.p + o;
-  const long doff[3] = {1, dy, dz};
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    if (!interior(g, c, i, j, k)) continue;
-    const T rhs = comp_rhs<T>(c, g, X0, X1, X2, dy, dz);
-    // pressure_gradient_{u,v,w} (include/PressureGradient.h:9-25)
-    const T pg = (p[0] - p[-doff[c]]) * (T)g.one_over_h[c];
-    const T x = ((const T*)s.X[c])[o];
-    if (STAGE == 1) {
-      ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs - (T)s.b * pg;
-      ((T*)s.O2[c])[o] = rhs;
-    } else if (STAGE == 2) {
-      const T rhs_1 = ((const T*)s.A[c])[o];
-      const T rhs_2_scaled = (T)s.a2 * rhs;
-      ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs_1 + rhs_2_scaled - (T)s.b * pg;
-      ((T*)s.O2[c])[o] = rhs_2_scaled;
-    } else {
-      const T rhs_2_scaled = -((const T*)s.A[c])[o];
-      ((T*)s.O1[c])[o] = x + rhs_2_scaled + (T)s.a2 * rhs - (T)s.b * pg;
-    }
-  }
-}
+constexpr int STX = 64, STY = 8, SHX = STX + 2, SHY = STY + 2, SKB = 16;
 
-// ---------------------------------------------------------------------------
-// Velocity-only RK3 stages with forcing (src/TimestepVelocity.cpp:20-51).
-//   Y2: R = N(X)+f(t_n);  B = X + dt*a21*R            (X = u, R -> rhs, B -> buf)
-//   Y3: R' = V + dt*(b1*R); V = V + dt*(a31*R + a32*(N(X)+f(t1)))  (X = buf)
-//   U*: B = R + dt*(b3*(N(X)+f(t2)))                  (X = u)
-// ---------------------------------------------------------------------------
-template <typename T, int STAGE>
-__global__ void __launch_bounds__(256) k_vel_stage(DGrid g, Layout L, VelStageArgs s, int imax, int jmax, int kmax) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > imax || j > jmax || k > kmax) return;
-  const long o = L.idx(i, j, k);
-  const long dy = L.PX, dz = L.plane;
-  const T* X0 = (const T*)s.X[0] + o;
-  const T* X1 = (const T*)s.X[1] + o;
-  const T* X2 = (const T*)s.X[2] + o;
-  const T dt = (T)s.dt;
-  for (int c = 0; c < 3; c++) {
-    if (!interior(g, c, i, j, k)) continue;
-    // calculate_momentum_rhs_with_forcing_* (include/MomentumEquationForcing.h:10-31)
-    const T f = comp_rhs<T>(c, g, X0, X1, X2, dy, dz) + (T)eval_at(g, s.forcing, c, c, s.t, i, j, k);
-    if (STAGE == 1) {
-      ((T*)s.R[c])[o] = f;
-      ((T*)s.B[c])[o] = ((const T*)s.X[c])[o] + dt * (T)(8.0 / 15.0) * f;
-    } else if (STAGE == 2) {
-      const T r = ((T*)s.R[c])[o];
-      const T vv = ((T*)s.V[c])[o];
-      ((T*)s.R[c])[o] = vv + dt * ((T)(1.0 / 4.0) * r);
-      ((T*)s.V[c])[o] = vv + dt * ((T)(1.0 / 4.0) * r + (T)(5.0 / 12.0) * f);
-    } else {
-      const T r = ((T*)s.R[c])[o];
-      ((T*)s.B[c])[o] = r + dt * ((T)(3.0 / 4.0) * f);
+template <typename T, int KIND>
+__global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax,
+                                               int kmax) {
+  __shared__ T pl[3][3][SHY][SHX];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * STX + tx;
+  const int i0 = blockIdx.x * STX + 1, j0 = blockIdx.y * STY + 1, k0 = blockIdx.z * SKB + 1;
+  const int k1 = min(k0 + SKB, kmax + 1);
+  const int i = i0 + tx, j = j0 + ty;
+  const T* X[3];
+  for (int f = 0; f < 3; f++) X[f] = (const T*)(KIND <= 3 ? s.X[f] : vs.X[f]);
+  auto load_plane = [&](int k, int slot) {
+    for (int q = tid; q < 3 * SHY * SHX; q += STX * STY) {
+      const int f = q / (SHY * SHX), r = q - f * (SHY * SHX);
+      const int jj = r / SHX, ii = r - jj * SHX;
+      const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+      T v = (T)0;
+      if (gi < L.PX && gj < L.SY && k < L.SZ) v = X[f][L.idx(gi, gj, k)];
+      pl[slot][f][jj][ii] = v;
     }
+  };
+  int sm = (k0 - 1) % 3, s0 = k0 % 3, sp = (k0 + 1) % 3;
+  load_plane(k0 - 1, sm);
+  load_plane(k0, s0);
+  const bool in_tile = i <= imax && j <= jmax;
+  for (int k = k0; k < k1; k++) {
+    load_plane(k + 1, sp);
+    __syncthreads();
+    if (in_tile) {
+      const int slot[3] = {sm, s0, sp};
+      auto tap = [&](int f, int di, int dj, int dk) -> T { return pl[slot[dk + 1]][f][ty + 1 + dj][tx + 1 + di]; };
+      const long o = L.idx(i, j, k);
+      const long doff[3] = {1, L.PX, L.plane};
+      for (int c = 0; c < 3; c++) {
+        if (!interior(g, c, i, j, k)) continue;
+        const T rhs = c == 0 ? rhs_u<T>(g, tap) : (c == 1 ? rhs_v<T>(g, tap) : rhs_w<T>(g, tap));
+        const T x = tap(c, 0, 0, 0);
+        if constexpr (KIND <= 3) {
+          // pressure_gradient_{u,v,w} (include/PressureGradient.h:9-25)
+          const T* p = (const T*)s.p + o;
+          const T pg = (p[0] - p[-doff[c]]) * (T)g.one_over_h[c];
+          if constexpr (KIND == 1) {
+            ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs - (T)s.b * pg;
+            ((T*)s.O2[c])[o] = rhs;
+          } else if constexpr (KIND == 2) {
+            const T rhs_1 = ((const T*)s.A[c])[o];
+            const T rhs_2_scaled = (T)s.a2 * rhs;
+            ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs_1 + rhs_2_scaled - (T)s.b * pg;
+            ((T*)s.O2[c])[o] = rhs_2_scaled;
+          } else {
+            const T rhs_2_scaled = -((const T*)s.A[c])[o];
+            ((T*)s.O1[c])[o] = x + rhs_2_scaled + (T)s.a2 * rhs - (T)s.b * pg;
+          }
+        } else {
+          // calculate_momentum_rhs_with_forcing_* (include/MomentumEquationForcing.h:10-31)
+          const T dt = (T)vs.dt;
+          const T f = rhs + (T)eval_at(g, vs.forcing, c, c, vs.t, i, j, k);
+          if constexpr (KIND == 4) {
+            ((T*)vs.R[c])[o] = f;
+            ((T*)vs.B[c])[o] = x + dt * (T)(8.0 / 15.0) * f;
+          } else if constexpr (KIND == 5) {
+            const T r = ((T*)vs.R[c])[o];
+            const T vv = ((T*)vs.V[c])[o];
+            ((T*)vs.R[c])[o] = vv + dt * ((T)(1.0 / 4.0) * r);
+            ((T*)vs.V[c])[o] = vv + dt * ((T)(1.0 / 4.0) * r + (T)(5.0 / 12.0) * f);
+          } else {
+            const T r = ((T*)vs.R[c])[o];
+            ((T*)vs.B[c])[o] = r + dt * ((T)(3.0 / 4.0) * f);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const int t = sm;
+    sm = s0;
+    s0 = sp;
+    sp = t;
   }
 }
 
@@ -325,10 +340,10 @@ __global__ void __launch_bounds__(256) k_set(DGrid g, Layout L, T* F, int stag, 
 // (i,j,k) -> (i-olo0, j-olo1, k-olo2), PressureTensor::copy_from_staggered,
 // src/PressureTensor.cpp:16-19).
 // ---------------------------------------------------------------------------
-template <typename T, typename S>
+template <typename T, typename S, bool NHN>
 __global__ void __launch_bounds__(256) k_div_rhs(DGrid g, Layout L, const T* __restrict__ u, const T* __restrict__ v,
                                                  const T* __restrict__ w, S* __restrict__ B, long bp0, long bplane,
-                                                 double dt, FnDesc gp, double tg, int nhn) {
+                                                 double dt, FnDesc gp, double tg) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;  // packed owner coordinates
   const int b = blockIdx.y;
   const int c = blockIdx.z;
@@ -339,7 +354,7 @@ __global__ void __launch_bounds__(256) k_div_rhs(DGrid g, Layout L, const T* __r
   const double dv_dy = ((double)v[o + L.PX] - (double)v[o]) * g.one_over_h[1];
   const double dw_dz = ((double)w[o + L.plane] - (double)w[o]) * g.one_over_h[2];
   double r = (du_dx + dv_dy + dw_dz) / dt;
-  if (nhn) {
+  if (NHN) {
     const int sx = g.N[0], sy = g.N[1], sz = g.N[2];
     if (k == 0) r += eval_at(g, gp, 3, 2, tg, i, j, k) * 2.0 * g.one_over_h[2];
     if (k == sz - 1) r -= eval_at(g, gp, 3, 2, tg, i, j, k) * 2.0 * g.one_over_h[2];
@@ -551,8 +566,9 @@ __global__ void __launch_bounds__(256) k_reduce_final(const Red3* partial, int n
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-template <typename T>
-void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st) {
+template <typename T, int KIND>
+static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& s, const VelStageArgs& vs,
+                              hipStream_t st) {
   // union of the three components' interiors
   int eu[3], ev[3], ew[3];
   extents(g, 0, eu);
@@ -561,33 +577,31 @@ void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s
   const int imax = max(eu[0], max(ev[0], ew[0])) - 2;
   const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
   const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
-  dim3 blk(64, 4, 1);
-  dim3 grd((imax + 63) / 64, (jmax + 3) / 4, kmax);
-  if (stage == 1)
-    k_stage<T, 1><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
-  else if (stage == 2)
-    k_stage<T, 2><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
-  else
-    k_stage<T, 3><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+  dim3 blk(STX, STY, 1);
+  dim3 grd((imax + STX - 1) / STX, (jmax + STY - 1) / STY, (kmax + SKB - 1) / SKB);
+  k_stage<T, KIND><<<grd, blk, 0, st>>>(g, L, s, vs, imax, jmax, kmax);
 }
 
 template <typename T>
-void launch_vel_stage(int stage, const DGrid& g, const Layout& L, const VelStageArgs& s, hipStream_t st) {
-  int eu[3], ev[3], ew[3];
-  extents(g, 0, eu);
-  extents(g, 1, ev);
-  extents(g, 2, ew);
-  const int imax = max(eu[0], max(ev[0], ew[0])) - 2;
-  const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
-  const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
-  dim3 blk(64, 4, 1);
-  dim3 grd((imax + 63) / 64, (jmax + 3) / 4, kmax);
+void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st) {
+  VelStageArgs vs{};
   if (stage == 1)
-    k_vel_stage<T, 1><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+    launch_stage_kind<T, 1>(g, L, s, vs, st);
   else if (stage == 2)
-    k_vel_stage<T, 2><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+    launch_stage_kind<T, 2>(g, L, s, vs, st);
   else
-    k_vel_stage<T, 3><<<grd, blk, 0, st>>>(g, L, s, imax, jmax, kmax);
+    launch_stage_kind<T, 3>(g, L, s, vs, st);
+}
+
+template <typename T>
+void launch_vel_stage(int stage, const DGrid& g, const Layout& L, const VelStageArgs& vs, hipStream_t st) {
+  StageArgs s{};
+  if (stage == 1)
+    launch_stage_kind<T, 4>(g, L, s, vs, st);
+  else if (stage == 2)
+    launch_stage_kind<T, 5>(g, L, s, vs, st);
+  else
+    launch_stage_kind<T, 6>(g, L, s, vs, st);
 }
 
 static void shell_counts(const DGrid& g, int stag, int cnt[3]) {
@@ -636,7 +650,10 @@ template <typename T, typename S>
 void launch_div_rhs(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, S* B, long bp0, long bplane,
                     double dt, const FnDesc& gp, double tg, int nhn, hipStream_t st) {
   dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
-  k_div_rhs<T, S><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg, nhn);
+  if (nhn)
+    k_div_rhs<T, S, true><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg);
+  else
+    k_div_rhs<T, S, false><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg);
 }
 
 template <typename T, typename S>

@@ -40,7 +40,8 @@ FN_CONSTANT = 11
 FN_TIME_DIFFERENCE = 12
 
 # mif_kernel_class
-KERNEL_CLASSES = ["stage", "bc", "div_rhs", "fft_x", "fft_y", "fft_z_fused", "unpack", "project", "defect"]
+KERNEL_CLASSES = ["stage", "bc", "div_rhs", "fft_x", "fft_y", "fft_z_fused", "unpack", "project", "defect",
+                  "fft_x_div", "fft_x_unpack"]
 
 
 class MifGrid(ctypes.Structure):

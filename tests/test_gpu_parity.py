@@ -27,7 +27,10 @@ POISSON_CASES = [
     ((8, 12, 11), (2 * PI, 2 * PI, 2 * PI), (0.0, 0.0, 0.0), (0, 0, 1)),  # walls x/y, periodic z
     ((33, 17, 33), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),      # channel
     ((23, 9, 14), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 1, 1)),          # generic radices (L = 44, 8, 13)
-    ((65, 65, 65), (2 * PI, 2.0, 2 * PI), (0.0, -1.0, 0.0), (1, 0, 1)),  # power-of-two fast path
+    ((65, 65, 65), (2 * PI, 2.0, 2 * PI), (0.0, -1.0, 0.0), (1, 0, 1)),  # power-of-two path L = 64, 128
+    ((257, 129, 33), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),    # L = 256 (x, y)
+    ((129, 257, 129), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),   # L = 128, 512, 128
+    ((65, 33, 129), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),        # Neumann L = 128, 64, 256
 ]
 
 
@@ -153,3 +156,95 @@ def test_timestep_velocity_matches_oracle():
     n_ref = o.error_norms_velocity(vel, ev, T)
     n_gpu = mif.error_norms_velocity(gvel, gev, T)
     assert np.allclose(n_gpu, n_ref, rtol=1e-9, atol=0)
+
+
+def test_full_timestep_channel_pow2_fused_path():
+    # power-of-two x length: divergence fused into the x forward pass and the
+    # x inverse pass writing the pressure field with its periodic images
+    _print_case("timestep channel 65x33x65")
+    o, vel, p, gvel, gp = _run_full((65, 33, 65), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 5e-3, 3, (1, 0, 1),
+                                    "channel", mo.FN_ZERO, seed=5)
+    for c in range(3):
+        assert rel_err(gvel[c], vel[c]) < TOL, c
+    assert rel_err(gp, p) < TOL
+
+
+def test_full_timestep_walls_pow2_fused_path():
+    _print_case("timestep ES 33^3 walls")
+    o, vel, p, gvel, gp = _run_full((33, 33, 33), (1.0, 1.0, 2.0), (0.0, 0.0, -1.0), 1e3, 1e-4, 2, (0, 0, 0), "es",
+                                    mo.FN_ES_VELOCITY)
+    for c in range(3):
+        assert rel_err(gvel[c], vel[c]) < TOL, c
+    assert rel_err(gp, p) < TOL
+
+
+def test_timestep_nhn_matches_oracle():
+    _print_case("timestep_nhn ES 16^3 walls")
+    Re, T, nsteps = 1e3, 1e-4, 2
+    Ng, size, origin = (16, 16, 16), (1.0, 1.0, 2.0), (0.0, 0.0, -1.0)
+    og, gs = grids(Ng, size, origin, Re, T, nsteps, (0, 0, 0))
+    o = mo.Oracle(og)
+    ev = mo.make_fn(mo.FN_ES_VELOCITY, Re=Re)
+    gpf = mo.make_fn(mo.FN_ES_GRAD_P, Re=Re)
+    vel, buf, buf2 = o.velocity(), o.velocity(), o.velocity()
+    p, pb = o.zeros(3), o.zeros(3)
+    o.set_velocity(vel, ev, 0.0)
+    o.set(p, 3, mo.make_fn(mo.FN_ES_PRESSURE, Re=Re), 0, 0.0)
+    ctx = mif.Context(gs)
+    gvel, gbuf, gbuf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    gp, gpb = ctx.pressure(), ctx.pressure()
+    gvel.upload(vel)
+    gp.upload(p)
+    gev, ggp = mif.function(mif.FN_ES_VELOCITY, Re=Re), mif.function(mif.FN_ES_GRAD_P, Re=Re)
+    for s in range(nsteps):
+        o.timestep(vel, buf, buf2, ev, s * o.dt, p, pb, grad_p=gpf)
+        mif.timestep_nhn(gvel, gbuf, gbuf2, gev, ggp, s * o.dt, gp, gpb)
+    for c, got in enumerate(gvel.download()):
+        assert rel_err(got, vel[c]) < TOL, c
+    assert rel_err(gp.download(), p) < TOL
+
+
+@pytest.mark.parametrize("Ng,per", [((33, 33, 33), (0, 0, 0)), ((17, 13, 11), (0, 0, 1))])
+def test_solve_pressure_and_adjust_and_norms(Ng, per):
+    # stand-alone pressure test of the reference (test/pressure_test_hn.cpp,
+    # test/pressure_test_mixed.cpp): dt = 1, [0, 2pi]^3
+    _print_case(f"solve_pressure {Ng} periodic={per}")
+    og, gs = grids(Ng, (2 * PI,) * 3, (0.0, 0.0, 0.0), 1.0, 1.0, 1, per)
+    o = mo.Oracle(og)
+    vel = o.velocity()
+    o.set_velocity(vel, mo.make_fn(mo.FN_PT_VELOCITY), 1.0)
+    p = o.zeros(3)
+    o.solve_pressure(p, vel, o.dt)
+    pe = mo.make_fn(mo.FN_PT_PRESSURE)
+    ctx = mif.Context(gs)
+    gvel = ctx.velocity()
+    gvel.set(mif.function(mif.FN_PT_VELOCITY), 1.0)
+    gpres = ctx.pressure()
+    mif.solve_pressure_equation_homogeneous_periodic(gpres, gvel, ctx.dt)
+    assert rel_err(gpres.download(), p) < TOL
+    o.adjust_pressure(p, pe, 1.0)
+    mif.adjust_pressure(gpres, mif.function(mif.FN_PT_PRESSURE), 1.0)
+    assert rel_err(gpres.download(), p) < TOL
+    n_ref = o.error_norms_pressure(p, pe, 1.0)
+    n_gpu = mif.error_norms_pressure(gpres, mif.function(mif.FN_PT_PRESSURE), 1.0)
+    assert np.allclose(n_gpu, n_ref, rtol=1e-9)
+
+
+def test_solve_pressure_nhn_matches_oracle():
+    # test/pressure_test_nhn.cpp: [-pi/2, 0]^3, Neumann data from grad p
+    _print_case("solve_pressure_nhn 17x13x11")
+    og, gs = grids((17, 13, 11), (PI / 2,) * 3, (-PI / 2,) * 3, 1.0, 1.0, 1, (0, 0, 0))
+    o = mo.Oracle(og)
+    vel = o.velocity()
+    o.set_velocity(vel, mo.make_fn(mo.FN_PT_VELOCITY), 1.0)
+    p = o.zeros(3)
+    # the oracle takes the gradient as a time-difference descriptor frozen at t = 1
+    gfix = mo.make_fn(mo.FN_TIME_DIFFERENCE, base=mo.FN_PT_GRAD_P, t0=-1.0, t1=1.0)
+    # base(1) - base(-1) = 2 * grad p(1) for this field (linear in t); halve it
+    o.solve_pressure(p, vel, o.dt, grad_p=gfix)
+    ctx = mif.Context(gs)
+    gvel = ctx.velocity()
+    gvel.set(mif.function(mif.FN_PT_VELOCITY), 1.0)
+    gpres = ctx.pressure()
+    mif.solve_pressure_equation_non_homogeneous_neumann(gpres, gvel, mif.function(mif.FN_PT_GRAD_P), 2.0, ctx.dt)
+    assert rel_err(gpres.download(), p) < TOL
