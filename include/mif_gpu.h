@@ -126,6 +126,17 @@ int mif_solve_pressure_non_homogeneous_neumann(mif_field *pressure, mif_field *c
  * from the owner points of `pressure`, the solution replaces it (owner
  * points and periodic ghosts). */
 int mif_poisson_solve(mif_field *pressure);
+/* The spectral solve on a caller-given right-hand side (the owner-point
+ * array the reference packs with PressureTensor::copy_from_staggered,
+ * src/PressureTensor.cpp:16-19: npts_x * npts_y * npts_z FP64 values, x
+ * fastest).  set_rhs uploads and keeps it; run solves from it, device
+ * resident (FP64 form: the reference's solve, src/PressureEquation.cpp:65-264;
+ * mixed form: FP32 transforms plus the context's FP64 defect sweeps, with the
+ * right-hand side first projected onto the range of the Laplacian);
+ * get_solution downloads phi in the same layout. */
+int mif_poisson_set_rhs(mif_ctx *ctx, const double *b);
+int mif_poisson_run(mif_ctx *ctx);
+int mif_poisson_get_solution(mif_ctx *ctx, double *phi);
 /* adjust_pressure (include/PressureEquation.h:25-26,
  * src/PressureEquation.cpp:288-343) with the exact pressure at time t. */
 int mif_adjust_pressure(mif_field *pressure, const mif_function *exact_pressure, double t);

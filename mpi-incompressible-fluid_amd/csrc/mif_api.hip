@@ -50,6 +50,7 @@ struct mif_ctx {
   void* B = nullptr;       // packed working buffer (field precision)
   double* B64 = nullptr;   // mixed form: FP64 right-hand side
   double* P64 = nullptr;   // mixed form: FP64 accumulated solution
+  double* R64 = nullptr;   // stand-alone Poisson runs: stored FP64 right-hand side
   // transfers and reductions
   double* staging = nullptr;
   long staging_elems = 0;
@@ -202,6 +203,13 @@ int check_vel(mif_field* const v[3], mif_ctx** out) {
 // forward x, forward y, fused z (forward, scale, inverse), inverse y,
 // inverse x.
 // ---------------------------------------------------------------------------
+int reduce_blocks(long total) {
+  long b = (total + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return (int)b;
+}
+
 template <typename T>
 SpectralScale scale_of(const mif_ctx* c) {
   SpectralScale sc;
@@ -291,11 +299,82 @@ int pressure_solve(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt,
   return MIF_OK;
 }
 
+// Mixed form with an FP64 right-hand side in c->B64 (owner points, packed):
+// project it onto the range of the Laplacian, solve in FP32, then `sweeps`
+// FP64 defect corrections; the result is left in c->P64 (sweeps > 0) or in
+// the FP32 buffer c->B (sweeps == 0).  Returns which one holds it.
+enum SolutionIn { IN_B32, IN_P64 };
+
+int mixed_solve_from_b64(mif_ctx* c, SolutionIn* where) {
+  float* B32 = (float*)c->B;
+  const long total = (long)c->g.npts[0] * c->g.npts[1] * c->g.npts[2];
+  const int nb = reduce_blocks(total);
+  {
+    KTimer t(c, MIF_KC_DEFECT);
+    launch_project_rhs(c->g, c->B64, B32, c->bp0, c->bplane, c->partial, nb, c->red_out, c->stream);
+  }
+  int rc = spectral_solve<float>(c, B32);
+  if (rc) return rc;
+  if (c->sweeps == 0) {
+    *where = IN_B32;
+    return MIF_OK;
+  }
+  {
+    KTimer t(c, MIF_KC_DEFECT);
+    launch_to64(c->g, B32, c->P64, c->bp0, c->bplane, c->stream);
+  }
+  for (int s = 0; s < c->sweeps; s++) {
+    {
+      KTimer t(c, MIF_KC_DEFECT);
+      launch_residual(c->g, c->B64, c->P64, B32, c->bp0, c->bplane, c->stream);
+    }
+    if ((rc = spectral_solve<float>(c, B32))) return rc;
+    {
+      KTimer t(c, MIF_KC_DEFECT);
+      launch_accumulate(c->g, c->P64, B32, c->bp0, c->bplane, c->stream);
+    }
+  }
+  {
+    KTimer t(c, MIF_KC_DEFECT);
+    launch_remove_mean(c->g, c->P64, c->bp0, c->bplane, c->partial, nb, c->red_out, c->stream);
+  }
+  LAUNCHCHK(c);
+  *where = IN_P64;
+  return MIF_OK;
+}
+
+// Pressure solve of the mixed form: FP64 divergence (plus Neumann data),
+// then mixed_solve_from_b64, then unpack into the FP32 pressure field.
+int pressure_solve_mixed(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
+                         int nhn) {
+  {
+    KTimer t(c, MIF_KC_DIV);
+    launch_div_rhs<float, double>(c->g, c->L, fptr<float>(vel[0]), fptr<float>(vel[1]), fptr<float>(vel[2]), c->B64,
+                                  c->bp0, c->bplane, dt, gp, tg, nhn, c->stream);
+  }
+  SolutionIn where;
+  int rc = mixed_solve_from_b64(c, &where);
+  if (rc) return rc;
+  {
+    KTimer t(c, MIF_KC_UNPACK);
+    if (where == IN_P64)
+      launch_unpack<float, double>(c->g, c->L, c->P64, c->bp0, c->bplane, fptr<float>(p), c->stream);
+    else
+      launch_unpack<float, float>(c->g, c->L, (float*)c->B, c->bp0, c->bplane, fptr<float>(p), c->stream);
+  }
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
 int pressure_solve_any(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
                        int nhn) {
   if (is_fp64(c)) return pressure_solve<double>(c, p, vel, dt, gp, tg, nhn);
+  if (c->sweeps > 0 || nhn) return pressure_solve_mixed(c, p, vel, dt, gp, tg, nhn);
   return pressure_solve<float>(c, p, vel, dt, gp, tg, nhn);
 }
+
+int pressure_solve_any(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
+                       int nhn);
 
 template <typename T>
 int stage(mif_ctx* c, int st, mif_field* const X[3], mif_field* const A[3], mif_field* const O1[3],
@@ -361,7 +440,7 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   apply_bc<T>(c, buf, ev, time_1);
   gd.t0 = time_1;
   gd.t1 = t_n;
-  if ((rc = pressure_solve<T>(c, pb, buf, dt_1, gd, 0.0, nhn))) return rc;
+  if ((rc = pressure_solve_any(c, pb, buf, dt_1, gd, 0.0, nhn))) return rc;
   project<T>(c, buf, p, pb, dt_1);
   // Stage 2 (src/Timestep.cpp:119-129)
   {
@@ -371,7 +450,7 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   apply_bc<T>(c, buf2, ev, time_2);
   gd.t0 = time_2;
   gd.t1 = time_1;
-  if ((rc = pressure_solve<T>(c, pb, buf2, dt_2, gd, 0.0, nhn))) return rc;
+  if ((rc = pressure_solve_any(c, pb, buf2, dt_2, gd, 0.0, nhn))) return rc;
   project<T>(c, buf2, p, pb, dt_2);
   // Stage 3 (src/Timestep.cpp:131-140)
   {
@@ -381,7 +460,7 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   apply_bc<T>(c, vel, ev, final_time);
   gd.t0 = final_time;
   gd.t1 = time_2;
-  if ((rc = pressure_solve<T>(c, pb, vel, dt_3, gd, 0.0, nhn))) return rc;
+  if ((rc = pressure_solve_any(c, pb, vel, dt_3, gd, 0.0, nhn))) return rc;
   project<T>(c, vel, p, pb, dt_3);
   LAUNCHCHK(c);
   return MIF_OK;
@@ -428,13 +507,6 @@ int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[
   for (int q = 0; q < 3; q++) std::swap(vel[q]->data, buf[q]->data);
   LAUNCHCHK(c);
   return MIF_OK;
-}
-
-int reduce_blocks(long total) {
-  long b = (total + 255) / 256;
-  if (b < 1) b = 1;
-  if (b > 1024) b = 1024;
-  return (int)b;
 }
 
 // Constants, layout, transform plans and scratch of a new context.
@@ -584,6 +656,7 @@ int mif_ctx_destroy(mif_ctx* c) {
   (void)hipFree(c->B);
   (void)hipFree(c->B64);
   (void)hipFree(c->P64);
+  (void)hipFree(c->R64);
   for (int a = 0; a < 3; a++) {
     (void)hipFree(c->W64[a]);
     (void)hipFree(c->W32[a]);
@@ -815,6 +888,47 @@ int mif_poisson_solve(mif_field* p) {
     launch_unpack<float, float>(c->g, c->L, (float*)c->B, c->bp0, c->bplane, fptr<float>(p), c->stream);
   }
   LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int mif_poisson_set_rhs(mif_ctx* c, const double* b) {
+  if (!c || !b) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  const long n0 = c->g.npts[0], n1 = c->g.npts[1], n2 = c->g.npts[2];
+  if (!c->R64) HIPCHK(c, hipMalloc(&c->R64, c->belems * sizeof(double)));
+  HIPCHK(c, hipMemsetAsync(c->R64, 0, c->belems * sizeof(double), c->stream));
+  HIPCHK(c, hipMemcpy2DAsync(c->R64, c->bp0 * sizeof(double), b, n0 * sizeof(double), n0 * sizeof(double), n1 * n2,
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MIF_OK;
+}
+
+int mif_poisson_run(mif_ctx* c) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  if (!c->R64) return fail(c, MIF_ERR_INVALID_ARGUMENT, "no right-hand side set");
+  const size_t bytes = c->belems * sizeof(double);
+  if (is_fp64(c)) {
+    HIPCHK(c, hipMemcpyAsync(c->B, c->R64, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return spectral_solve<double>(c, (double*)c->B);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->B64, c->R64, bytes, hipMemcpyDeviceToDevice, c->stream));
+  SolutionIn where;
+  int rc = mixed_solve_from_b64(c, &where);
+  if (rc) return rc;
+  if (where == IN_B32) {
+    KTimer t(c, MIF_KC_DEFECT);
+    launch_to64(c->g, (const float*)c->B, c->P64, c->bp0, c->bplane, c->stream);
+  }
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+int mif_poisson_get_solution(mif_ctx* c, double* phi) {
+  if (!c || !phi) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  const long n0 = c->g.npts[0], n1 = c->g.npts[1], n2 = c->g.npts[2];
+  const double* src = is_fp64(c) ? (const double*)c->B : c->P64;
+  HIPCHK(c, hipMemcpy2DAsync(phi, n0 * sizeof(double), src, c->bp0 * sizeof(double), n0 * sizeof(double), n1 * n2,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MIF_OK;
 }
 

@@ -112,4 +112,15 @@ template <typename T>
 bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
                      long bplane, const FusedIO* io, hipStream_t st);
 
+// Mixed form: FP64 right-hand side projection and defect correction
+// (mif_mixed.hip).
+void launch_project_rhs(const DGrid& g, double* B64, float* B32, long bp0, long bplane, Red3* partial, int nblocks,
+                        Red3* total, hipStream_t st);
+void launch_to64(const DGrid& g, const float* B32, double* P, long bp0, long bplane, hipStream_t st);
+void launch_remove_mean(const DGrid& g, double* P, long bp0, long bplane, Red3* partial, int nblocks, Red3* total,
+                        hipStream_t st);
+void launch_residual(const DGrid& g, const double* b, const double* P, float* B32, long bp0, long bplane,
+                     hipStream_t st);
+void launch_accumulate(const DGrid& g, double* P, const float* B32, long bp0, long bplane, hipStream_t st);
+
 }  // namespace mif

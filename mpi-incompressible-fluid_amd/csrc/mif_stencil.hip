@@ -754,5 +754,8 @@ template void launch_pack<float, float>(const DGrid&, const Layout&, const float
 template void launch_unpack<double, double>(const DGrid&, const Layout&, const double*, long, long, double*,
                                             hipStream_t);
 template void launch_unpack<float, float>(const DGrid&, const Layout&, const float*, long, long, float*, hipStream_t);
+template void launch_unpack<float, double>(const DGrid&, const Layout&, const double*, long, long, float*, hipStream_t);
+template void launch_unpack<double, float>(const DGrid&, const Layout&, const float*, long, long, double*, hipStream_t);
+template void launch_pack<float, double>(const DGrid&, const Layout&, const float*, double*, long, long, hipStream_t);
 
 }  // namespace mif

@@ -93,6 +93,9 @@ SIGNATURES = {
     "mif_solve_pressure_homogeneous_periodic": [_VP, _V3, _D],
     "mif_solve_pressure_non_homogeneous_neumann": [_VP, _V3, _FP, _D, _D],
     "mif_poisson_solve": [_VP],
+    "mif_poisson_set_rhs": [_VP, _DP],
+    "mif_poisson_run": [_VP],
+    "mif_poisson_get_solution": [_VP, _DP],
     "mif_adjust_pressure": [_VP, _FP, _D],
     "mif_error_norms_velocity": [_V3, _FP, _D, _DP],
     "mif_error_norms_pressure": [_VP, _FP, _D, _DP],

@@ -127,6 +127,29 @@ class Context:
     def pressure(self) -> "Field":
         return Field(self, L.STAGGER_NONE)
 
+    # stand-alone spectral solve on a packed owner-point right-hand side ---------
+    @property
+    def npts(self):
+        g = self.grid
+        Ng = (g.Nx_global, g.Ny_global, g.Nz_global)
+        return tuple(Ng[a] - (1 if g.periodic[a] else 0) for a in range(3))
+
+    def poisson_set_rhs(self, b: np.ndarray):
+        n = self.npts
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.shape != (n[2], n[1], n[0]):
+            raise ValueError(f"shape {b.shape} != {(n[2], n[1], n[0])}")
+        self.check(self.lib.mif_poisson_set_rhs(self.ptr, b.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def poisson_run(self):
+        self.check(self.lib.mif_poisson_run(self.ptr))
+
+    def poisson_get_solution(self) -> np.ndarray:
+        n = self.npts
+        out = np.empty((n[2], n[1], n[0]), dtype=np.float64)
+        self.check(self.lib.mif_poisson_get_solution(self.ptr, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     # timing -----------------------------------------------------------------
     def set_timing(self, on: bool):
         self.check(self.lib.mif_ctx_set_timing(self.ptr, 1 if on else 0))

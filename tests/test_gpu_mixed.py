@@ -1,0 +1,92 @@
+"""The mixed-precision form (FP32 fields, stencils and transforms; FP64
+right-hand side, null-space projection and defect correction) against the
+FP64 oracle: error bounds per field, and the defect sweeps' convergence."""
+import numpy as np
+import pytest
+
+import mif_oracle as mo
+from helpers import PI, channel_initial, grids, rel_err, seeded_normal
+from test_oracle_poisson import apply_op, weights
+
+pytestmark = pytest.mark.gpu
+
+mif = pytest.importorskip("mif_amd")
+
+
+def l2rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("Ng,per", [((65, 33, 65), (1, 0, 1)), ((33, 33, 33), (0, 0, 0)), ((129, 65, 129), (1, 0, 1))])
+def test_defect_correction_converges(Ng, per):
+    print(f"[case] mixed poisson {Ng} {per}", flush=True)
+    og, gs = grids(Ng, (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1.0, 1.0, 1, per)
+    o = mo.Oracle(og)
+    b = seeded_normal(tuple(reversed(o.npts)), 20261202)
+    W = weights(o).reshape(b.shape)
+    bp = b - (W * b).sum() / W.sum()
+    ref = b.copy()
+    o.poisson_packed(ref)
+    ctx = mif.Context(gs, precision=mif.MIXED)
+    ctx.poisson_set_rhs(b)
+    errs, res = [], []
+    for sweeps in (0, 1, 2):
+        ctx.set_defect_sweeps(sweeps)
+        ctx.poisson_run()
+        phi = ctx.poisson_get_solution()
+        errs.append(l2rel(phi, ref))
+        res.append(np.linalg.norm(bp - apply_op(o, phi)) / np.linalg.norm(bp))
+    print("errors", errs, "residuals", res)
+    assert errs[0] < 1e-5
+    assert errs[1] < errs[0] * 1e-2 and errs[2] < 1e-10
+    assert res[2] < 1e-11
+
+
+@pytest.mark.parametrize("sweeps", [0, 1])
+def test_mixed_timestep_channel_error(sweeps):
+    print(f"[case] mixed timestep channel sweeps={sweeps}", flush=True)
+    Ng, size, origin, Re, T, nsteps, per = (65, 33, 65), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 5e-3, 10, (1, 0, 1)
+    og, gs = grids(Ng, size, origin, Re, T, nsteps, per)
+    o = mo.Oracle(og)
+    vel, buf, buf2 = o.velocity(), o.velocity(), o.velocity()
+    p, pb = o.zeros(3), o.zeros(3)
+    v0 = channel_initial(o, 9)
+    for c in range(3):
+        vel[c][...] = v0[c]
+    ctx = mif.Context(gs, precision=mif.MIXED)
+    ctx.set_defect_sweeps(sweeps)
+    gvel, gbuf, gbuf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    gp, gpb = ctx.pressure(), ctx.pressure()
+    gvel.upload(vel)
+    ev = mo.make_fn(mo.FN_ZERO)
+    fn = mif.function(mif.FN_ZERO)
+    for s in range(nsteps):
+        o.timestep(vel, buf, buf2, ev, s * o.dt, p, pb)
+        mif.timestep(gvel, gbuf, gbuf2, fn, s * o.dt, gp, gpb)
+    got = gvel.download()
+    e = [l2rel(got[c], vel[c]) for c in range(3)]
+    ev_all = float(np.sqrt(sum(np.linalg.norm(got[c] - vel[c]) ** 2 for c in range(3)) /
+                           sum(np.linalg.norm(vel[c]) ** 2 for c in range(3))))
+    print("per-component", e, "velocity", ev_all, "pressure", l2rel(gp.download(), p))
+    assert ev_all < 1e-6
+
+
+def test_mixed_timestep_velocity_error():
+    print("[case] mixed timestep_velocity", flush=True)
+    Re = 1e4
+    og, gs = grids((33, 33, 33), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), Re, 1e-3, 5, (0, 0, 0))
+    o = mo.Oracle(og)
+    ev = mo.make_fn(mo.FN_VT_VELOCITY, Re=Re)
+    fo = mo.make_fn(mo.FN_VT_FORCING, Re=Re)
+    vel, buf, rhs = o.velocity(), o.velocity(), o.velocity()
+    o.set_velocity(vel, ev, 0.0)
+    ctx = mif.Context(gs, precision=mif.MIXED)
+    gvel, gbuf, grhs = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    gvel.upload(vel)
+    gev, gfo = mif.function(mif.FN_VT_VELOCITY, Re=Re), mif.function(mif.FN_VT_FORCING, Re=Re)
+    for s in range(5):
+        o.timestep_velocity(vel, buf, rhs, ev, fo, s * o.dt)
+        mif.timestep_velocity(gvel, gbuf, grhs, gev, gfo, s * o.dt)
+    got = gvel.download()
+    for c in range(3):
+        assert rel_err(got[c], vel[c]) < 1e-6, c

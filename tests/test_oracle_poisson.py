@@ -149,3 +149,25 @@ def test_manufactured_pressure_solution_converges():
         errs.append(o.error_norms_pressure(p, pe, 1.0)[1])
     r1, r2 = np.log2(errs[0] / errs[1]), np.log2(errs[1] / errs[2])
     assert 1.7 < r1 < 2.3 and 1.8 < r2 < 2.2, errs
+
+
+def test_nonhomogeneous_neumann_solution_converges():
+    # test/pressure_test_nhn.cpp: [-pi/2, 0]^3, walls everywhere, Neumann data
+    # from the exact gradient (apply_non_homogeneous_neumann,
+    # src/PressureEquation.cpp:10-57); the error converges at second order,
+    # which a wrong sign or a missing factor 2/h in the boundary rows breaks
+    errs = []
+    for n in (9, 17, 33):
+        og, _ = grids((n, n, n), (PI / 2,) * 3, (-PI / 2,) * 3, 1.0, 1.0, 1, (0, 0, 0))
+        o = mo.Oracle(og)
+        vel = o.velocity()
+        o.set_velocity(vel, mo.make_fn(mo.FN_PT_VELOCITY), 1.0)
+        p = o.zeros(3)
+        # the gradient at t = 1 as a time difference base(1) - base(0) (the field is linear in t)
+        gp = mo.make_fn(mo.FN_TIME_DIFFERENCE, base=mo.FN_PT_GRAD_P, t0=0.0, t1=1.0)
+        o.solve_pressure(p, vel, o.dt, grad_p=gp)
+        pe = mo.make_fn(mo.FN_PT_PRESSURE)
+        o.adjust_pressure(p, pe, 1.0)
+        errs.append(o.error_norms_pressure(p, pe, 1.0)[1])
+    r1, r2 = np.log2(errs[0] / errs[1]), np.log2(errs[1] / errs[2])
+    assert 1.6 < r1 < 2.4 and 1.8 < r2 < 2.2, errs
