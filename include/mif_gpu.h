@@ -77,6 +77,10 @@ int mif_field_extents(const mif_field *f, uint64_t extents[3]);
 int mif_field_upload(mif_field *f, const double *host);
 /* Copy the field to a host array in the reference layout. */
 int mif_field_download(const mif_field *f, double *host);
+/* The same two copies with a device array in the reference layout (FP64),
+ * e.g. a torch tensor's storage; synchronise the context's stream. */
+int mif_field_upload_device(mif_field *f, const double *device_src);
+int mif_field_download_device(const mif_field *f, double *device_dst);
 /* Tensor::swap_data (include/Tensor.h:107-109): exchange device storage. */
 int mif_field_swap(mif_field *a, mif_field *b);
 /* StaggeredTensor::set (src/StaggeredTensor.cpp:205-219) with a known
@@ -153,6 +157,24 @@ int mif_error_norms_pressure(const mif_field *pressure, const mif_function *exac
  * include/PressureGradient.h:9-25, as filled in test/full_test.cpp:141-142). */
 int mif_pressure_gradient(const mif_field *pressure, mif_field *const out[3]);
 
+/* ---- adjoint (no reference counterpart) -----------------------------------
+ * Constant body force f added to the momentum right-hand side N of every
+ * stage of mif_timestep (default 0, which is the reference's equation). */
+int mif_ctx_set_body_force(mif_ctx *ctx, const double force[3]);
+/* Adjoint of one mif_timestep (homogeneous pressure-increment conditions).
+ * un, y2, y3 are the velocity fields the forward step used as stencil
+ * sources: u^n, and the `buf` and `buf2` triples as mif_timestep leaves them
+ * (Y2 and Y3 after their projections).  On entry ubar and pbar hold the
+ * adjoints of (u^{n+1}, p^{n+1}); on return those of (u^n, p^n).
+ * force_bar[3] is incremented by the adjoint of the body force.  Dirichlet
+ * data are constants of the map.  Synchronises the stream. */
+int mif_timestep_adjoint(mif_field *const un[3], mif_field *const y2[3], mif_field *const y3[3],
+                         mif_field *const ubar[3], mif_field *pbar, double force_bar[3]);
+/* Device copy between fields of one staggering (Tensor copy). */
+int mif_field_copy(mif_field *dst, const mif_field *src);
+/* Sum over all points of the extents of a * b (FP64 accumulation). */
+int mif_field_dot(const mif_field *a, const mif_field *b, double *out);
+
 /* ---- measurement ------------------------------------------------------------
  * Kernel timing with HIP events recorded on the context's stream around
  * every launch of the kernels of one class (no reference counterpart; used
@@ -169,7 +191,8 @@ typedef enum mif_kernel_class {
   MIF_KC_DEFECT = 8,    /* mixed form: FP64 residual / correction          */
   MIF_KC_FFT_X_DIV = 9, /* divergence fused into the x forward pass        */
   MIF_KC_FFT_X_UNPACK = 10, /* x inverse pass writing the pressure field   */
-  MIF_KC_COUNT = 11
+  MIF_KC_ADJOINT = 11,  /* whole adjoint step (all its kernels)            */
+  MIF_KC_COUNT = 12
 } mif_kernel_class;
 int mif_ctx_set_timing(mif_ctx *ctx, int enable);
 /* Total milliseconds and launch count of a class since timing was enabled

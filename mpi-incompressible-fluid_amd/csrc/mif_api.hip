@@ -51,6 +51,13 @@ struct mif_ctx {
   double* B64 = nullptr;   // mixed form: FP64 right-hand side
   double* P64 = nullptr;   // mixed form: FP64 accumulated solution
   double* R64 = nullptr;   // stand-alone Poisson runs: stored FP64 right-hand side
+  double force[3]{};       // body force of timestep() (zero: the reference)
+  // adjoint workspace (allocated on first use): t1, t2, Sb, R1b, Nb triples,
+  // phib, per-block partial sums and the force-adjoint accumulator
+  void* adj[5][3]{};
+  void* adj_phib = nullptr;
+  double* adj_partial = nullptr;
+  double* adj_fbar = nullptr;
   // transfers and reductions
   double* staging = nullptr;
   long staging_elems = 0;
@@ -390,6 +397,7 @@ int stage(mif_ctx* c, int st, mif_field* const X[3], mif_field* const A[3], mif_
   s.a1 = a1;
   s.a2 = a2;
   s.b = b;
+  for (int q = 0; q < 3; q++) s.force[q] = c->force[q];
   KTimer t(c, MIF_KC_STAGE);
   launch_stage<T>(st, c->g, c->L, s, c->stream);
   return MIF_OK;
@@ -506,6 +514,80 @@ int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[
   // velocity.swap_data(velocity_buffer) (src/TimestepVelocity.cpp:89)
   for (int q = 0; q < 3; q++) std::swap(vel[q]->data, buf[q]->data);
   LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Adjoint of one step (no reference counterpart).  The reverse sweep of
+// FULL_TIMESTEP (src/Timestep.cpp:97-141): per stage, in reverse order,
+//   phib = pbar + dt_s G^T ubar                  (projection, p += phi)
+//   ubar += D^T W M W^-1 U^T phib / dt_s         (Poisson solve transposed)
+//   Y    = B^T ubar                              (boundary map transposed)
+//   Nb, Sb/R1b from Y; Xb = Y + J_N(X)^T Nb      (stage update transposed)
+//   pbar += b_s G^T Y;  fbar += sum Nb
+// ---------------------------------------------------------------------------
+int adj_workspace(mif_ctx* c) {
+  if (c->adj_phib) return MIF_OK;
+  const size_t bytes = c->L.elems * c->esize;
+  for (int q = 0; q < 5; q++)
+    for (int cc = 0; cc < 3; cc++) {
+      HIPCHK(c, hipMalloc(&c->adj[q][cc], bytes));
+      HIPCHK(c, hipMemsetAsync(c->adj[q][cc], 0, bytes, c->stream));
+    }
+  HIPCHK(c, hipMalloc(&c->adj_phib, bytes));
+  HIPCHK(c, hipMemsetAsync(c->adj_phib, 0, bytes, c->stream));
+  const long blocks = (long)((c->g.N[0] + 1 + 255) / 256) * (c->g.N[1] + 1) * (c->g.N[2] + 1);
+  HIPCHK(c, hipMalloc(&c->adj_partial, 3 * blocks * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->adj_fbar, 3 * sizeof(double)));
+  return MIF_OK;
+}
+
+AdjVec vec_of(mif_field* const f[3]) { return AdjVec{{f[0]->data, f[1]->data, f[2]->data}}; }
+AdjVec vec_ws(mif_ctx* c, int q) { return AdjVec{{c->adj[q][0], c->adj[q][1], c->adj[q][2]}}; }
+
+template <typename T>
+int step_adjoint(mif_ctx* c, mif_field* const un[3], mif_field* const y2[3], mif_field* const y3[3],
+                 mif_field* const ubar[3], mif_field* pbar, double fbar[3]) {
+  int rc = adj_workspace(c);
+  if (rc) return rc;
+  const DGrid& g = c->g;
+  const Layout& L = c->L;
+  hipStream_t st = c->stream;
+  const double dt = g.dt;
+  const double dts[4] = {0, 64.0 / 120.0 * dt, 16.0 / 120.0 * dt, 40.0 / 120.0 * dt};
+  const double s1_a1 = 64.0 / 120.0 * dt, s1_b = s1_a1;
+  const double s2_a1 = -34.0 / 120.0 * dt, s2_a2 = 50.0 / 120.0 * dt, s2_b = s2_a1 + s2_a2;
+  const double s3_a2 = -50.0 / 120.0 * dt, s3_a3 = 90.0 / 120.0 * dt, s3_b = s3_a2 + s3_a3;
+  const double bcoef[4] = {0, s1_b, s2_b, s3_b};
+  const AdjVec T1 = vec_ws(c, 0), T2 = vec_ws(c, 1), SB = vec_ws(c, 2), R1B = vec_ws(c, 3), NB = vec_ws(c, 4);
+  const AdjVec UB = vec_of(ubar);
+  const AdjVec src[4] = {AdjVec{}, vec_of(un), vec_of(y2), vec_of(y3)};
+  T* P = fptr<T>(pbar);
+  T* phib = (T*)c->adj_phib;
+  T* B = (T*)c->B;
+  HIPCHK(c, hipMemsetAsync(c->adj_fbar, 0, 3 * sizeof(double), st));
+  KTimer tm(c, MIF_KC_ADJOINT);
+  for (int s = 3; s >= 1; s--) {
+    // the adjoint of the velocity after this stage's projection lives in
+    // ubar (stage 3) or T2 (stages 2, 1)
+    const AdjVec Yin = s == 3 ? UB : T2;
+    launch_adj_grad<T>(g, L, P, phib, Yin, dts[s], st);
+    launch_adj_unpack<T>(g, L, phib, B, c->bp0, c->bplane, st);
+    if ((rc = spectral_solve<T>(c, B))) return rc;
+    launch_adj_div<T>(g, L, B, c->bp0, c->bplane, Yin, dts[s], st);
+    launch_adj_bc<T>(g, L, Yin, T1, st);
+    const int nblk = launch_adj_nbar<T>(s, g, L, T1, NB, SB, R1B, s == 2 ? s2_a1 : s1_a1, s2_a2, s3_a3, c->adj_partial,
+                                        st);
+    launch_sum3(c->adj_partial, nblk, c->adj_fbar, st);
+    launch_adj_convdiff<T>(g, L, src[s], NB, T1, s == 1 ? UB : T2, st);
+    launch_adj_grad<T>(g, L, P, P, T1, bcoef[s], st);
+  }
+  LAUNCHCHK(c);
+  double fb[3];
+  HIPCHK(c, hipMemcpyAsync(fb, c->adj_fbar, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  for (int q = 0; q < 3; q++) fbar[q] += fb[q];
   return MIF_OK;
 }
 
@@ -657,6 +739,11 @@ int mif_ctx_destroy(mif_ctx* c) {
   (void)hipFree(c->B64);
   (void)hipFree(c->P64);
   (void)hipFree(c->R64);
+  for (int q = 0; q < 5; q++)
+    for (int cc = 0; cc < 3; cc++) (void)hipFree(c->adj[q][cc]);
+  (void)hipFree(c->adj_phib);
+  (void)hipFree(c->adj_partial);
+  (void)hipFree(c->adj_fbar);
   for (int a = 0; a < 3; a++) {
     (void)hipFree(c->W64[a]);
     (void)hipFree(c->W32[a]);
@@ -762,6 +849,34 @@ int mif_field_download(const mif_field* f, double* host) {
     launch_to_dense<float>(c->L, e, (const float*)f->data, c->staging, c->stream);
   LAUNCHCHK(c);
   HIPCHK(c, hipMemcpyAsync(host, c->staging, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MIF_OK;
+}
+
+int mif_field_upload_device(mif_field* f, const double* dev) {
+  if (!f || !dev) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  mif_ctx* c = f->ctx;
+  int e[3];
+  extents(c->g, f->stag, e);
+  if (is_fp64(c))
+    launch_from_dense<double>(c->L, e, dev, fptr<double>(f), c->stream);
+  else
+    launch_from_dense<float>(c->L, e, dev, fptr<float>(f), c->stream);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MIF_OK;
+}
+
+int mif_field_download_device(const mif_field* f, double* dev) {
+  if (!f || !dev) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  mif_ctx* c = f->ctx;
+  int e[3];
+  extents(c->g, f->stag, e);
+  if (is_fp64(c))
+    launch_to_dense<double>(c->L, e, (const double*)f->data, dev, c->stream);
+  else
+    launch_to_dense<float>(c->L, e, (const float*)f->data, dev, c->stream);
+  LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MIF_OK;
 }
@@ -1023,6 +1138,53 @@ int mif_pressure_gradient(const mif_field* p, mif_field* const out[3]) {
   return MIF_OK;
 }
 
+int mif_ctx_set_body_force(mif_ctx* c, const double force[3]) {
+  if (!c || !force) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  for (int q = 0; q < 3; q++) c->force[q] = force[q];
+  return MIF_OK;
+}
+
+int mif_timestep_adjoint(mif_field* const un[3], mif_field* const y2[3], mif_field* const y3[3],
+                         mif_field* const ubar[3], mif_field* pbar, double force_bar[3]) {
+  mif_ctx *c = nullptr, *c2 = nullptr, *c3 = nullptr, *c4 = nullptr;
+  int rc;
+  if ((rc = check_vel(un, &c)) || (rc = check_vel(y2, &c2)) || (rc = check_vel(y3, &c3)) ||
+      (rc = check_vel(ubar, &c4)))
+    return rc;
+  if (c2 != c || c3 != c || c4 != c || !pbar || pbar->ctx != c || pbar->stag != MIF_STAGGER_NONE || !force_bar)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "inconsistent adjoint arguments");
+  if (is_fp64(c)) return step_adjoint<double>(c, un, y2, y3, ubar, pbar, force_bar);
+  return step_adjoint<float>(c, un, y2, y3, ubar, pbar, force_bar);
+}
+
+int mif_field_copy(mif_field* dst, const mif_field* src) {
+  if (!dst || !src || dst->ctx != src->ctx || dst->stag != src->stag)
+    return fail(dst ? dst->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad fields");
+  mif_ctx* c = dst->ctx;
+  HIPCHK(c, hipMemcpyAsync(dst->data, src->data, c->L.elems * c->esize, hipMemcpyDeviceToDevice, c->stream));
+  return MIF_OK;
+}
+
+int mif_field_dot(const mif_field* a, const mif_field* b, double* out) {
+  if (!a || !b || !out || a->ctx != b->ctx || a->stag != b->stag)
+    return fail(a ? a->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad fields");
+  mif_ctx* c = a->ctx;
+  int e[3];
+  extents(c->g, a->stag, e);
+  if (is_fp64(c))
+    launch_dot<double>(c->L, e, (const double*)a->data, (const double*)b->data, c->partial, c->nred, c->red_out,
+                       c->stream);
+  else
+    launch_dot<float>(c->L, e, (const float*)a->data, (const float*)b->data, c->partial, c->nred, c->red_out,
+                      c->stream);
+  LAUNCHCHK(c);
+  Red3 r;
+  HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(Red3), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = r.s1;
+  return MIF_OK;
+}
+
 int mif_ctx_set_timing(mif_ctx* c, int enable) {
   if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
   drain_timers(c);
@@ -1043,8 +1205,8 @@ int mif_ctx_kernel_time(mif_ctx* c, int cls, double* total_ms, uint64_t* launche
 }
 
 const char* mif_kernel_class_name(int cls) {
-  static const char* names[MIF_KC_COUNT] = {"stage",  "bc",      "div_rhs", "fft_x",     "fft_y",       "fft_z_fused",
-                                            "unpack", "project", "defect",  "fft_x_div", "fft_x_unpack"};
+  static const char* names[MIF_KC_COUNT] = {"stage",  "bc",      "div_rhs", "fft_x",     "fft_y",        "fft_z_fused",
+                                            "unpack", "project", "defect",  "fft_x_div", "fft_x_unpack", "adjoint"};
   if (cls < 0 || cls >= MIF_KC_COUNT) return "unknown";
   return names[cls];
 }

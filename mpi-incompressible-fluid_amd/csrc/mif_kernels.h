@@ -16,6 +16,7 @@ struct StageArgs {
   void* O2[3];
   const void* p;
   double a1, a2, b;
+  double force[3];  // constant body force added to N (zero in the reference)
 };
 
 // Arguments of the velocity-only stage kernel (src/TimestepVelocity.cpp:20-51).
@@ -91,6 +92,9 @@ void launch_to_dense(const Layout& L, const int e[3], const T* src, double* dst,
 template <typename T>
 void launch_copy(long n, const T* src, T* dst, hipStream_t st);
 template <typename T>
+void launch_dot(const Layout& L, const int e[3], const T* a, const T* b, Red3* partial, int nblocks, Red3* out,
+                hipStream_t st);
+template <typename T>
 void launch_add_const(const DGrid& g, const Layout& L, T* P, double c, hipStream_t st);
 template <typename T>
 void launch_norm_velocity(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, const FnDesc& f,
@@ -111,6 +115,30 @@ void launch_fft_axis(const DGrid& g, int axis, int dir, int fused, const FftPlan
 template <typename T>
 bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
                      long bplane, const FusedIO* io, hipStream_t st);
+
+// Three component arrays of one velocity-shaped quantity.
+struct AdjVec {
+  void* f[3];
+};
+
+// Adjoint of one step (mif_adjoint.hip).
+template <typename T>
+void launch_adj_grad(const DGrid& g, const Layout& L, const T* in, T* out, const AdjVec& Ob, double coef,
+                     hipStream_t st);
+template <typename T>
+void launch_adj_unpack(const DGrid& g, const Layout& L, const T* phib, T* B, long bp0, long bplane, hipStream_t st);
+template <typename T>
+void launch_adj_div(const DGrid& g, const Layout& L, const T* B, long bp0, long bplane, const AdjVec& yb, double dts,
+                    hipStream_t st);
+template <typename T>
+void launch_adj_bc(const DGrid& g, const Layout& L, const AdjVec& yb, const AdjVec& Yb, hipStream_t st);
+template <typename T>
+int launch_adj_nbar(int stage, const DGrid& g, const Layout& L, const AdjVec& Y, const AdjVec& Nb, const AdjVec& Sb,
+                    const AdjVec& R1b, double a1, double a2, double a3, double* partial, hipStream_t st);
+template <typename T>
+void launch_adj_convdiff(const DGrid& g, const Layout& L, const AdjVec& X, const AdjVec& Nb, const AdjVec& id,
+                         const AdjVec& Xb, hipStream_t st);
+void launch_sum3(const double* partial, long nblocks, double* out, hipStream_t st);
 
 // Mixed form: FP64 right-hand side projection and defect correction
 // (mif_mixed.hip).

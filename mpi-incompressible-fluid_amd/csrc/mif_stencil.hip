@@ -115,9 +115,10 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
       const long doff[3] = {1, L.PX, L.plane};
       for (int c = 0; c < 3; c++) {
         if (!interior(g, c, i, j, k)) continue;
-        const T rhs = c == 0 ? rhs_u<T>(g, tap) : (c == 1 ? rhs_v<T>(g, tap) : rhs_w<T>(g, tap));
+        T rhs = c == 0 ? rhs_u<T>(g, tap) : (c == 1 ? rhs_v<T>(g, tap) : rhs_w<T>(g, tap));
         const T x = tap(c, 0, 0, 0);
         if constexpr (KIND <= 3) {
+          rhs += (T)s.force[c];
           // pressure_gradient_{u,v,w} (include/PressureGradient.h:9-25)
           const T* p = (const T*)s.p + o;
           const T pg = (p[0] - p[-doff[c]]) * (T)g.one_over_h[c];
@@ -556,6 +557,21 @@ __global__ void __launch_bounds__(256) k_norm_pressure(DGrid g, Layout L, const 
   if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
+// sum over a field's extents of a * b (FP64 accumulation)
+template <typename T>
+__global__ void __launch_bounds__(256) k_dot(Layout L, int e0, int e1, int e2, const T* __restrict__ a,
+                                             const T* __restrict__ b, Red3* partial) {
+  const long total = (long)e0 * e1 * e2;
+  Red3 acc{0, 0, 0};
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % e0), j = (int)((q / e0) % e1), k = (int)(q / ((long)e0 * e1));
+    const long o = L.idx(i, j, k);
+    acc.s1 += (double)a[o] * (double)b[o];
+  }
+  const Red3 r = block_reduce(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
 __global__ void __launch_bounds__(256) k_reduce_final(const Red3* partial, int n, Red3* out) {
   Red3 acc{0, 0, 0};
   for (int q = threadIdx.x; q < n; q += blockDim.x) acc = red_combine(acc, partial[q]);
@@ -723,6 +739,13 @@ void launch_norm_pressure(const DGrid& g, const Layout& L, const T* p, const FnD
   k_reduce_final<<<1, 256, 0, st>>>(partial, nblocks, out);
 }
 
+template <typename T>
+void launch_dot(const Layout& L, const int e[3], const T* a, const T* b, Red3* partial, int nblocks, Red3* out,
+                hipStream_t st) {
+  k_dot<T><<<nblocks, 256, 0, st>>>(L, e[0], e[1], e[2], a, b, partial);
+  k_reduce_final<<<1, 256, 0, st>>>(partial, nblocks, out);
+}
+
 #define MIF_INSTANTIATE(T)                                                                                          \
   template void launch_stage<T>(int, const DGrid&, const Layout&, const StageArgs&, hipStream_t);                   \
   template void launch_vel_stage<T>(int, const DGrid&, const Layout&, const VelStageArgs&, hipStream_t);            \
@@ -734,6 +757,7 @@ void launch_norm_pressure(const DGrid& g, const Layout& L, const T* p, const FnD
   template void launch_from_dense<T>(const Layout&, const int*, const double*, T*, hipStream_t);                    \
   template void launch_to_dense<T>(const Layout&, const int*, const T*, double*, hipStream_t);                      \
   template void launch_copy<T>(long, const T*, T*, hipStream_t);                                                    \
+  template void launch_dot<T>(const Layout&, const int*, const T*, const T*, Red3*, int, Red3*, hipStream_t);       \
   template void launch_add_const<T>(const DGrid&, const Layout&, T*, double, hipStream_t);                          \
   template void launch_norm_velocity<T>(const DGrid&, const Layout&, const T*, const T*, const T*, const FnDesc&,   \
                                         double, Red3*, int, Red3*, hipStream_t);                                    \

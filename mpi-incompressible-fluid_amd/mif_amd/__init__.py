@@ -11,6 +11,6 @@ from ._lib import (FP64, MIXED, STAGGER_NONE, STAGGER_X, STAGGER_Y, STAGGER_Z, F
 from .core import (Context, Field, GridSpec, MifError, VelocityField, adjust_pressure, error_norms_pressure,
                    error_norms_velocity, function, poisson_solve, pressure_gradient,
                    solve_pressure_equation_homogeneous_periodic, solve_pressure_equation_non_homogeneous_neumann,
-                   timestep, timestep_nhn, timestep_velocity)
+                   timestep, timestep_adjoint, timestep_nhn, timestep_velocity)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

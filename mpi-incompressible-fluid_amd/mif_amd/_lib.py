@@ -41,7 +41,7 @@ FN_TIME_DIFFERENCE = 12
 
 # mif_kernel_class
 KERNEL_CLASSES = ["stage", "bc", "div_rhs", "fft_x", "fft_y", "fft_z_fused", "unpack", "project", "defect",
-                  "fft_x_div", "fft_x_unpack"]
+                  "fft_x_div", "fft_x_unpack", "adjoint"]
 
 
 class MifGrid(ctypes.Structure):
@@ -83,6 +83,8 @@ SIGNATURES = {
     "mif_field_extents": [_VP, _U64P],
     "mif_field_upload": [_VP, _DP],
     "mif_field_download": [_VP, _DP],
+    "mif_field_upload_device": [_VP, _VP],
+    "mif_field_download_device": [_VP, _VP],
     "mif_field_swap": [_VP, _VP],
     "mif_field_set": [_VP, _FP, _I, _D, _I],
     "mif_field_apply_periodic_bc": [_VP],
@@ -100,6 +102,10 @@ SIGNATURES = {
     "mif_error_norms_velocity": [_V3, _FP, _D, _DP],
     "mif_error_norms_pressure": [_VP, _FP, _D, _DP],
     "mif_pressure_gradient": [_VP, _V3],
+    "mif_ctx_set_body_force": [_VP, _DP],
+    "mif_timestep_adjoint": [_V3, _V3, _V3, _V3, _VP, _DP],
+    "mif_field_copy": [_VP, _VP],
+    "mif_field_dot": [_VP, _VP, _DP],
     "mif_ctx_set_timing": [_VP, _I],
     "mif_ctx_kernel_time": [_VP, _I, _DP, _U64P],
     "mif_kernel_class_name": [_I],

@@ -1,0 +1,123 @@
+"""Differentiable rollout of the timestep behind torch.autograd.Function.
+
+forward(u0, f) runs `nsteps` full RK3 timesteps (mif_timestep) from the
+initial velocity u0 (three FP64 device tensors in the reference layout,
+shape (sz, sy, sx) per component) with a constant body force f (3,) and
+returns u(T).  It keeps a checkpoint of (u, p) every `checkpoint_every`
+steps.  backward replays each checkpoint interval once, keeping the three
+stencil sources of every step (u^n and the two stage buffers), and runs the
+hand-written adjoint of each step (mif_timestep_adjoint) in reverse.  Memory:
+one (u, p) per checkpoint plus 3 velocity triples per step of one interval.
+
+No counterpart in the reference (it has no adjoint); the forward map is the
+reference's timestep() (src/Timestep.cpp:97-156) with a body force added to
+the momentum right-hand side.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .core import Context, VelocityField, function, timestep, timestep_adjoint
+
+
+class Rollout:
+    """nsteps of timestep() on one context, differentiable w.r.t. the initial
+    velocity and a constant body force."""
+
+    def __init__(self, ctx: Context, nsteps: int, checkpoint_every: int = 8, exact_velocity=None):
+        if ctx.precision != L.FP64 and ctx.precision != L.MIXED:
+            raise ValueError("unknown precision")
+        self.ctx = ctx
+        self.nsteps = int(nsteps)
+        self.k = max(1, int(checkpoint_every))
+        self.ev = exact_velocity if exact_velocity is not None else function(L.FN_ZERO)
+        self.vel, self.buf, self.buf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+        self.p, self.pb = ctx.pressure(), ctx.pressure()
+        ncp = (self.nsteps + self.k - 1) // self.k
+        self.cp_u = [ctx.velocity() for _ in range(ncp)]
+        self.cp_p = [ctx.pressure() for _ in range(ncp)]
+        self.rec = [(ctx.velocity(), ctx.velocity(), ctx.velocity()) for _ in range(min(self.k, self.nsteps))]
+        self.ubar, self.pbar = ctx.velocity(), ctx.pressure()
+        self.shapes = [f.shape for f in self.vel.components]
+
+    # -- device transfers with torch tensors -----------------------------------
+    def _load(self, vf: VelocityField, tensors):
+        torch.cuda.synchronize()
+        for f, t in zip(vf.components, tensors):
+            t = t.detach().contiguous()
+            assert t.dtype == torch.float64 and t.is_cuda and tuple(t.shape) == f.shape
+            f.upload_device(t.data_ptr())
+
+    def _store(self, vf: VelocityField):
+        out = []
+        for f in vf.components:
+            t = torch.empty(f.shape, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            f.download_device(t.data_ptr())
+            out.append(t)
+        return out
+
+    def _zero(self, field):
+        field.upload(np.zeros(field.shape))
+
+    def _step(self, s: int):
+        timestep(self.vel, self.buf, self.buf2, self.ev, s * self.ctx.dt, self.p, self.pb)
+
+    # -- forward / backward -----------------------------------------------------
+    def run_forward(self, u0, f):
+        self.ctx.set_body_force([float(x) for x in f])
+        self._load(self.vel, u0)
+        self._zero(self.p)
+        for s in range(self.nsteps):
+            if s % self.k == 0:
+                c = s // self.k
+                self.cp_u[c].copy_from(self.vel)
+                self.cp_p[c].copy_from(self.p)
+            self._step(s)
+        self.ctx.synchronize()
+        return self._store(self.vel)
+
+    def run_backward(self, grad_uT):
+        self._load(self.ubar, grad_uT)
+        self._zero(self.pbar)
+        fbar = np.zeros(3)
+        ncp = len(self.cp_u)
+        for c in reversed(range(ncp)):
+            s0 = c * self.k
+            s1 = min(self.nsteps, s0 + self.k)
+            self.vel.copy_from(self.cp_u[c])
+            self.p.copy_from(self.cp_p[c])
+            for s in range(s0, s1):
+                un, y2, y3 = self.rec[s - s0]
+                un.copy_from(self.vel)
+                self._step(s)
+                y2.copy_from(self.buf)
+                y3.copy_from(self.buf2)
+            for s in reversed(range(s0, s1)):
+                un, y2, y3 = self.rec[s - s0]
+                timestep_adjoint(un, y2, y3, self.ubar, self.pbar, fbar)
+        self.ctx.synchronize()
+        return self._store(self.ubar), fbar
+
+    def __call__(self, u0, f):
+        return _RolloutFn.apply(self, f, *u0)
+
+
+class _RolloutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx_, roll: Rollout, f, u, v, w):
+        ctx_.roll = roll
+        out = roll.run_forward((u, v, w), f.detach().cpu().numpy())
+        return tuple(out)
+
+    @staticmethod
+    def backward(ctx_, gu, gv, gw):
+        roll = ctx_.roll
+        grads = []
+        for g, shp in zip((gu, gv, gw), roll.shapes):
+            grads.append(torch.zeros(shp, dtype=torch.float64, device="cuda") if g is None else g)
+        gu0, fbar = roll.run_backward(grads)
+        gf = torch.tensor(fbar, dtype=torch.float64)
+        return (None, gf, *gu0)

@@ -117,6 +117,12 @@ class Context:
     def set_defect_sweeps(self, n: int):
         self.check(self.lib.mif_ctx_set_defect_sweeps(self.ptr, n))
 
+    def set_body_force(self, f):
+        """Constant body force added to N in every stage of timestep()
+        (zero = the reference's equation)."""
+        arr = (ctypes.c_double * 3)(*[float(x) for x in f])
+        self.check(self.lib.mif_ctx_set_body_force(self.ptr, arr))
+
     # fields -----------------------------------------------------------------
     def field(self, staggering: int) -> "Field":
         return Field(self, staggering)
@@ -203,6 +209,22 @@ class Field:
         self.ctx.check(self.ctx.lib.mif_field_download(self.ptr, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def upload_device(self, ptr: int):
+        """Copy from a device array (reference layout, FP64) at address ptr."""
+        self.ctx.check(self.ctx.lib.mif_field_upload_device(self.ptr, ctypes.c_void_p(ptr)))
+
+    def download_device(self, ptr: int):
+        """Copy into a device array (reference layout, FP64) at address ptr."""
+        self.ctx.check(self.ctx.lib.mif_field_download_device(self.ptr, ctypes.c_void_p(ptr)))
+
+    def copy_from(self, other: "Field"):
+        self.ctx.check(self.ctx.lib.mif_field_copy(self.ptr, other.ptr))
+
+    def dot(self, other: "Field") -> float:
+        out = ctypes.c_double()
+        self.ctx.check(self.ctx.lib.mif_field_dot(self.ptr, other.ptr, ctypes.byref(out)))
+        return out.value
+
     def set(self, fn: L.MifFunction, comp: int = 0, t: float = 0.0, include_border: bool = True):
         self.ctx.check(self.ctx.lib.mif_field_set(self.ptr, ctypes.byref(fn), comp, t, 1 if include_border else 0))
 
@@ -243,6 +265,13 @@ class VelocityField:
     def swap_data(self, other: "VelocityField"):
         for a, b in zip(self.components, other.components):
             a.swap_data(b)
+
+    def copy_from(self, other: "VelocityField"):
+        for a, b in zip(self.components, other.components):
+            a.copy_from(b)
+
+    def dot(self, other: "VelocityField") -> float:
+        return sum(a.dot(b) for a, b in zip(self.components, other.components))
 
 
 # ---- reference-named free functions -------------------------------------------
@@ -308,6 +337,19 @@ def error_norms_pressure(pressure: Field, exact_pressure: L.MifFunction, t: floa
     out = (ctypes.c_double * 3)()
     pressure.ctx.check(pressure.ctx.lib.mif_error_norms_pressure(pressure.ptr, ctypes.byref(exact_pressure), t, out))
     return np.array(list(out))
+
+
+def timestep_adjoint(u_n: VelocityField, y2: VelocityField, y3: VelocityField, u_bar: VelocityField,
+                     p_bar: Field, force_bar: np.ndarray):
+    """Adjoint of one timestep(): (u_bar, p_bar) go from the adjoints of
+    (u^{n+1}, p^{n+1}) to those of (u^n, p^n); force_bar (3) accumulates.
+    u_n, y2, y3 are the stencil sources of the forward step (u^n and the
+    buffers `velocity_buffer`, `velocity_buffer_2` as timestep() leaves them)."""
+    ctx = u_n.ctx
+    fb = (ctypes.c_double * 3)(*[float(x) for x in force_bar])
+    ctx.check(ctx.lib.mif_timestep_adjoint(u_n.c_array(), y2.c_array(), y3.c_array(), u_bar.c_array(), p_bar.ptr, fb))
+    for a in range(3):
+        force_bar[a] = fb[a]
 
 
 def pressure_gradient(pressure: Field, out: VelocityField):

@@ -842,7 +842,7 @@ void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3],
       for (long k = 1; k < e[2] - 1; k++)
         for (long j = 1; j < e[1] - 1; j++)
           for (long i = 1; i < e[0] - 1; i++) {
-            const double rhs = orc_momentum_rhs(g, vel, c, i, j, k);
+            const double rhs = orc_momentum_rhs(g, vel, c, i, j, k) + g->force[c];
             const double pg = pgrad(g, p, c, i, j, k);
             AT(buf[c], e, i, j, k) = AT(vel[c], e, i, j, k) + a1 * rhs - b * pg;
             AT(buf2[c], e, i, j, k) = rhs;
@@ -871,7 +871,7 @@ void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3],
         for (long j = 1; j < e[1] - 1; j++)
           for (long i = 1; i < e[0] - 1; i++) {
             const double rhs_1 = AT(buf2[c], e, i, j, k);
-            const double rhs_2 = orc_momentum_rhs(g, buf, c, i, j, k);
+            const double rhs_2 = orc_momentum_rhs(g, buf, c, i, j, k) + g->force[c];
             const double rhs_2_scaled = a2 * rhs_2;
             const double pg = pgrad(g, p, c, i, j, k);
             AT(buf2[c], e, i, j, k) = AT(buf[c], e, i, j, k) + a1 * rhs_1 + rhs_2_scaled - b * pg;
@@ -899,7 +899,7 @@ void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3],
         for (long j = 1; j < e[1] - 1; j++)
           for (long i = 1; i < e[0] - 1; i++) {
             const double rhs_2_scaled = -AT(vel[c], e, i, j, k);
-            const double rhs_3 = orc_momentum_rhs(g, buf2, c, i, j, k);
+            const double rhs_3 = orc_momentum_rhs(g, buf2, c, i, j, k) + g->force[c];
             const double pg = pgrad(g, p, c, i, j, k);
             AT(vel[c], e, i, j, k) = AT(buf2[c], e, i, j, k) + rhs_2_scaled + a3 * rhs_3 - b * pg;
           }
@@ -1065,4 +1065,326 @@ void orc_pressure_gradient(const orc_grid *g, const double *p, double *const out
       for (long j = 1; j < e[1] - 1; j++)
         for (long i = 1; i < e[0] - 1; i++) AT(out[c], e, i, j, k) = pgrad(g, p, c, i, j, k);
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of one timestep() (no reference counterpart).                     */
+/*                                                                          */
+/* The step is a composition of: the stage updates (N(X) = momentum RHS,    */
+/* quadratic in X), the boundary map (Dirichlet faces = constants, periodic */
+/* ghosts = copies of interior points), the divergence, the spectral solve  */
+/* M, the unpack with periodic images, p += phi and the interior projection */
+/* u -= dt_s G phi.  Each linear piece is transposed as a scatter here; the */
+/* spectral solve's transpose is W M W^-1 with W the trapezoid weights (the */
+/* DCT-I matrix is diag(c) times a symmetric matrix, c = 1 at the two ends  */
+/* and 2 inside).  Serial scatters keep the oracle simple and deterministic. */
+/* ------------------------------------------------------------------------ */
+static int interior_pt(const orc_grid *g, int c, long i, long j, long k) {
+  long e[3];
+  orc_extents(g, c, e);
+  return i >= 1 && j >= 1 && k >= 1 && i <= e[0] - 2 && j <= e[1] - 2 && k <= e[2] - 2;
+}
+
+static double *vzeros(const orc_grid *g, int s) { return (double *)calloc((size_t)orc_field_size(g, s), sizeof(double)); }
+
+/* Xb += J_N(X)^T Nb (Nb holds values at interior points of each component). */
+static void adj_convdiff(const orc_grid *g, double *const X[3], double *const Nb[3], double *const Xb[3]) {
+  long e[3][3];
+  for (int c = 0; c < 3; c++) orc_extents(g, c, e[c]);
+  const double *u = X[0], *v = X[1], *w = X[2];
+  double *ub = Xb[0], *vb = Xb[1], *wb = Xb[2];
+  const double Dx = g->one_over_h2Re[0], Dy = g->one_over_h2Re[1], Dz = g->one_over_h2Re[2];
+  const double A2 = g->one_over_2h[0], B2 = g->one_over_2h[1], C2 = g->one_over_2h[2];
+  const double A8 = g->one_over_8h[0], B8 = g->one_over_8h[1], C8 = g->one_over_8h[2];
+#define U(i, j, k) AT(u, e[0], i, j, k)
+#define V(i, j, k) AT(v, e[1], i, j, k)
+#define W(i, j, k) AT(w, e[2], i, j, k)
+#define UB(i, j, k) AT(ub, e[0], i, j, k)
+#define VB(i, j, k) AT(vb, e[1], i, j, k)
+#define WB(i, j, k) AT(wb, e[2], i, j, k)
+  /* u component */
+  for (long k = 1; k < e[0][2] - 1; k++)
+    for (long j = 1; j < e[0][1] - 1; j++)
+      for (long i = 1; i < e[0][0] - 1; i++) {
+        const double n = AT(Nb[0], e[0], i, j, k);
+        if (n == 0.0) continue;
+        const double dux = U(i + 1, j, k) - U(i - 1, j, k), duy = U(i, j + 1, k) - U(i, j - 1, k),
+                     duz = U(i, j, k + 1) - U(i, j, k - 1);
+        const double Vs = V(i, j + 1, k) + V(i, j, k) + V(i - 1, j + 1, k) + V(i - 1, j, k);
+        const double Ws = W(i, j, k + 1) + W(i, j, k) + W(i - 1, j, k + 1) + W(i - 1, j, k);
+        UB(i, j, k) += n * (-dux * A2 - 2 * (Dx + Dy + Dz));
+        UB(i + 1, j, k) += n * (-U(i, j, k) * A2 + Dx);
+        UB(i - 1, j, k) += n * (U(i, j, k) * A2 + Dx);
+        UB(i, j + 1, k) += n * (-Vs * B8 + Dy);
+        UB(i, j - 1, k) += n * (Vs * B8 + Dy);
+        UB(i, j, k + 1) += n * (-Ws * C8 + Dz);
+        UB(i, j, k - 1) += n * (Ws * C8 + Dz);
+        const double cv = -duy * B8, cw = -duz * C8;
+        VB(i, j + 1, k) += n * cv;
+        VB(i, j, k) += n * cv;
+        VB(i - 1, j + 1, k) += n * cv;
+        VB(i - 1, j, k) += n * cv;
+        WB(i, j, k + 1) += n * cw;
+        WB(i, j, k) += n * cw;
+        WB(i - 1, j, k + 1) += n * cw;
+        WB(i - 1, j, k) += n * cw;
+      }
+  /* v component */
+  for (long k = 1; k < e[1][2] - 1; k++)
+    for (long j = 1; j < e[1][1] - 1; j++)
+      for (long i = 1; i < e[1][0] - 1; i++) {
+        const double n = AT(Nb[1], e[1], i, j, k);
+        if (n == 0.0) continue;
+        const double dvx = V(i + 1, j, k) - V(i - 1, j, k), dvy = V(i, j + 1, k) - V(i, j - 1, k),
+                     dvz = V(i, j, k + 1) - V(i, j, k - 1);
+        const double Us = U(i + 1, j, k) + U(i, j, k) + U(i + 1, j - 1, k) + U(i, j - 1, k);
+        const double Ws = W(i, j, k + 1) + W(i, j, k) + W(i, j - 1, k + 1) + W(i, j - 1, k);
+        VB(i, j, k) += n * (-dvy * B2 - 2 * (Dx + Dy + Dz));
+        VB(i + 1, j, k) += n * (-Us * A8 + Dx);
+        VB(i - 1, j, k) += n * (Us * A8 + Dx);
+        VB(i, j + 1, k) += n * (-V(i, j, k) * B2 + Dy);
+        VB(i, j - 1, k) += n * (V(i, j, k) * B2 + Dy);
+        VB(i, j, k + 1) += n * (-Ws * C8 + Dz);
+        VB(i, j, k - 1) += n * (Ws * C8 + Dz);
+        const double cu = -dvx * A8, cw = -dvz * C8;
+        UB(i + 1, j, k) += n * cu;
+        UB(i, j, k) += n * cu;
+        UB(i + 1, j - 1, k) += n * cu;
+        UB(i, j - 1, k) += n * cu;
+        WB(i, j, k + 1) += n * cw;
+        WB(i, j, k) += n * cw;
+        WB(i, j - 1, k + 1) += n * cw;
+        WB(i, j - 1, k) += n * cw;
+      }
+  /* w component */
+  for (long k = 1; k < e[2][2] - 1; k++)
+    for (long j = 1; j < e[2][1] - 1; j++)
+      for (long i = 1; i < e[2][0] - 1; i++) {
+        const double n = AT(Nb[2], e[2], i, j, k);
+        if (n == 0.0) continue;
+        const double dwx = W(i + 1, j, k) - W(i - 1, j, k), dwy = W(i, j + 1, k) - W(i, j - 1, k),
+                     dwz = W(i, j, k + 1) - W(i, j, k - 1);
+        const double Us = U(i + 1, j, k) + U(i, j, k) + U(i + 1, j, k - 1) + U(i, j, k - 1);
+        const double Vs = V(i, j + 1, k) + V(i, j, k) + V(i, j + 1, k - 1) + V(i, j, k - 1);
+        WB(i, j, k) += n * (-dwz * C2 - 2 * (Dx + Dy + Dz));
+        WB(i + 1, j, k) += n * (-Us * A8 + Dx);
+        WB(i - 1, j, k) += n * (Us * A8 + Dx);
+        WB(i, j + 1, k) += n * (-Vs * B8 + Dy);
+        WB(i, j - 1, k) += n * (Vs * B8 + Dy);
+        WB(i, j, k + 1) += n * (-W(i, j, k) * C2 + Dz);
+        WB(i, j, k - 1) += n * (W(i, j, k) * C2 + Dz);
+        const double cu = -dwx * A8, cv = -dwy * B8;
+        UB(i + 1, j, k) += n * cu;
+        UB(i, j, k) += n * cu;
+        UB(i + 1, j, k - 1) += n * cu;
+        UB(i, j, k - 1) += n * cu;
+        VB(i, j + 1, k) += n * cv;
+        VB(i, j, k) += n * cv;
+        VB(i, j + 1, k - 1) += n * cv;
+        VB(i, j, k - 1) += n * cv;
+      }
+#undef U
+#undef V
+#undef W
+#undef UB
+#undef VB
+#undef WB
+}
+
+/* pb += G^T applied to Ob with coefficient b: the forward term of component
+ * c at interior Q is -b (p(Q) - p(Q - e_c)) / h_c. */
+static void adj_grad(const orc_grid *g, double *const Ob[3], double b, double *pb) {
+  long ep[3];
+  orc_extents(g, 3, ep);
+  const long off[3] = {1, ep[0], ep[0] * ep[1]};
+  for (int c = 0; c < 3; c++) {
+    long e[3];
+    orc_extents(g, c, e);
+    for (long k = 1; k < e[2] - 1; k++)
+      for (long j = 1; j < e[1] - 1; j++)
+        for (long i = 1; i < e[0] - 1; i++) {
+          const double o = AT(Ob[c], e, i, j, k) * b * g->one_over_h[c];
+          const long q = i + ep[0] * (j + ep[1] * k);
+          pb[q] -= o;
+          pb[q - off[c]] += o;
+        }
+  }
+}
+
+/* Source owner index of a point of the pressure extents (periodic map). */
+static long owner_of(const orc_grid *g, long i, long j, long k) {
+  long s[3] = {i, j, k};
+  for (int a = 0; a < 3; a++) {
+    if (!g->periodic[a]) continue;
+    if (s[a] == 0) s[a] = g->N[a] - 2;
+    else if (s[a] == g->N[a] - 1) s[a] = 1;
+    s[a] -= 1;
+  }
+  return s[0] + g->npts[0] * (s[1] + g->npts[1] * s[2]);
+}
+
+static double trap_w(const orc_grid *g, int a, long i) {
+  if (g->periodic[a]) return 1.0;
+  return (i == 0 || i == g->npts[a] - 1) ? 0.5 : 1.0;
+}
+
+/* yb += (D^T P^T M^T U^T phib) / dts, M^T = W M W^-1. */
+static void adj_solve(const orc_grid *g, const double *phib, double dts, double *const yb[3]) {
+  long ep[3];
+  orc_extents(g, 3, ep);
+  const long *n = g->npts;
+  double *b = (double *)calloc((size_t)(n[0] * n[1] * n[2]), sizeof(double));
+  for (long k = 0; k < ep[2]; k++)
+    for (long j = 0; j < ep[1]; j++)
+      for (long i = 0; i < ep[0]; i++) b[owner_of(g, i, j, k)] += AT(phib, ep, i, j, k);
+  for (long k = 0; k < n[2]; k++)
+    for (long j = 0; j < n[1]; j++)
+      for (long i = 0; i < n[0]; i++)
+        b[i + n[0] * (j + n[1] * k)] /= trap_w(g, 0, i) * trap_w(g, 1, j) * trap_w(g, 2, k);
+  orc_poisson_packed(g, b);
+  for (long k = 0; k < n[2]; k++)
+    for (long j = 0; j < n[1]; j++)
+      for (long i = 0; i < n[0]; i++)
+        b[i + n[0] * (j + n[1] * k)] *= trap_w(g, 0, i) * trap_w(g, 1, j) * trap_w(g, 2, k) / dts;
+  long lo[3];
+  for (int a = 0; a < 3; a++) lo[a] = g->periodic[a] ? 1 : 0;
+  long e[3][3];
+  for (int c = 0; c < 3; c++) orc_extents(g, c, e[c]);
+  for (long k = 0; k < n[2]; k++)
+    for (long j = 0; j < n[1]; j++)
+      for (long i = 0; i < n[0]; i++) {
+        const double bb = b[i + n[0] * (j + n[1] * k)];
+        const long I = i + lo[0], J = j + lo[1], K = k + lo[2];
+        AT(yb[0], e[0], I + 1, J, K) += bb * g->one_over_h[0];
+        AT(yb[0], e[0], I, J, K) -= bb * g->one_over_h[0];
+        AT(yb[1], e[1], I, J + 1, K) += bb * g->one_over_h[1];
+        AT(yb[1], e[1], I, J, K) -= bb * g->one_over_h[1];
+        AT(yb[2], e[2], I, J, K + 1) += bb * g->one_over_h[2];
+        AT(yb[2], e[2], I, J, K) -= bb * g->one_over_h[2];
+      }
+  free(b);
+}
+
+/* Yb = B^T yb: an interior point collects its own adjoint and those of the
+ * periodic ghosts copied from it; boundary-shell entries become zero
+ * (Dirichlet values are constants of the map). */
+static void adj_bc(const orc_grid *g, double *const yb[3], double *const Yb[3]) {
+  for (int c = 0; c < 3; c++) {
+    long e[3];
+    orc_extents(g, c, e);
+    memset(Yb[c], 0, sizeof(double) * (size_t)orc_field_size(g, c));
+    for (long k = 1; k < e[2] - 1; k++)
+      for (long j = 1; j < e[1] - 1; j++)
+        for (long i = 1; i < e[0] - 1; i++) {
+          const long P[3] = {i, j, k};
+          long opts[3][3];
+          int nopt[3];
+          for (int a = 0; a < 3; a++) {
+            nopt[a] = 1;
+            opts[a][0] = P[a];
+            if (g->periodic[a]) {
+              if (P[a] == 1) opts[a][nopt[a]++] = e[a] - 1;
+              if (P[a] == e[a] - 2) opts[a][nopt[a]++] = 0;
+            }
+          }
+          double s = 0;
+          for (int qa = 0; qa < nopt[0]; qa++)
+            for (int qb = 0; qb < nopt[1]; qb++)
+              for (int qc = 0; qc < nopt[2]; qc++) s += AT(yb[c], e, opts[0][qa], opts[1][qb], opts[2][qc]);
+          AT(Yb[c], e, i, j, k) = s;
+        }
+  }
+}
+
+static void interior_scaled(const orc_grid *g, double *const src[3], double a, double *const dst[3]) {
+  for (int c = 0; c < 3; c++) {
+    long e[3];
+    orc_extents(g, c, e);
+    memset(dst[c], 0, sizeof(double) * (size_t)orc_field_size(g, c));
+    for (long k = 1; k < e[2] - 1; k++)
+      for (long j = 1; j < e[1] - 1; j++)
+        for (long i = 1; i < e[0] - 1; i++) AT(dst[c], e, i, j, k) = a * AT(src[c], e, i, j, k);
+  }
+}
+
+static void add_force_bar(const orc_grid *g, double *const Nb[3], double fbar[3]) {
+  for (int c = 0; c < 3; c++) {
+    long e[3];
+    orc_extents(g, c, e);
+    double s = 0;
+    for (long k = 1; k < e[2] - 1; k++)
+      for (long j = 1; j < e[1] - 1; j++)
+        for (long i = 1; i < e[0] - 1; i++) s += AT(Nb[c], e, i, j, k);
+    fbar[c] += s;
+  }
+}
+
+void orc_timestep_adjoint(const orc_grid *g, double *const un[3], double *const y2[3], double *const y3[3],
+                          double *const ubar[3], double *pbar, double fbar[3]) {
+  const double dt = g->dt;
+  const double dt_1 = 64.0 / 120.0 * dt, dt_2 = 16.0 / 120.0 * dt, dt_3 = 40.0 / 120.0 * dt;
+  const double s1_a1 = 64.0 / 120.0 * dt, s1_b = s1_a1;
+  const double s2_a1 = -34.0 / 120.0 * dt, s2_a2 = 50.0 / 120.0 * dt, s2_b = s2_a1 + s2_a2;
+  const double s3_a2 = -50.0 / 120.0 * dt, s3_a3 = 90.0 / 120.0 * dt, s3_b = s3_a2 + s3_a3;
+  double *t1[3], *t2[3], *Sb[3], *R1b[3], *Nb[3];
+  for (int c = 0; c < 3; c++) {
+    t1[c] = vzeros(g, c);
+    t2[c] = vzeros(g, c);
+    Sb[c] = vzeros(g, c);
+    R1b[c] = vzeros(g, c);
+    Nb[c] = vzeros(g, c);
+  }
+  const long np = orc_field_size(g, 3);
+  double *phib = (double *)malloc(sizeof(double) * (size_t)np);
+
+  /* stage 3: u^{n+1} = B(u') - dt_3 G phi_3, p^{n+1} = p_2 + phi_3 */
+  memcpy(phib, pbar, sizeof(double) * (size_t)np);
+  adj_grad(g, ubar, dt_3, phib);
+  adj_solve(g, phib, dt_3, ubar);
+  adj_bc(g, ubar, t1); /* t1 = adjoint of u' (interior) */
+  interior_scaled(g, t1, s3_a3, Nb);
+  interior_scaled(g, t1, 1.0, t2);
+  adj_convdiff(g, y3, Nb, t2); /* t2 = adjoint of Y3' */
+  interior_scaled(g, t1, -1.0, Sb);
+  adj_grad(g, t1, s3_b, pbar); /* pbar = adjoint of p_2 */
+  add_force_bar(g, Nb, fbar);
+
+  /* stage 2: Y3 = Y2' + a1 R1 + a2 N(Y2'), S = a2 N(Y2') */
+  memcpy(phib, pbar, sizeof(double) * (size_t)np);
+  adj_grad(g, t2, dt_2, phib);
+  adj_solve(g, phib, dt_2, t2);
+  adj_bc(g, t2, t1); /* t1 = adjoint of Y3 (interior) */
+  for (int c = 0; c < 3; c++) {
+    const long n = orc_field_size(g, c);
+    for (long q = 0; q < n; q++) Nb[c][q] = s2_a2 * (t1[c][q] + Sb[c][q]);
+  }
+  interior_scaled(g, t1, 1.0, t2);
+  adj_convdiff(g, y2, Nb, t2); /* t2 = adjoint of Y2' */
+  interior_scaled(g, t1, s2_a1, R1b);
+  adj_grad(g, t1, s2_b, pbar); /* pbar = adjoint of p_1 */
+  add_force_bar(g, Nb, fbar);
+
+  /* stage 1: Y2 = u^n + a1 N(u^n) - b G p, R1 = N(u^n) */
+  memcpy(phib, pbar, sizeof(double) * (size_t)np);
+  adj_grad(g, t2, dt_1, phib);
+  adj_solve(g, phib, dt_1, t2);
+  adj_bc(g, t2, t1); /* t1 = adjoint of Y2 (interior) */
+  for (int c = 0; c < 3; c++) {
+    const long n = orc_field_size(g, c);
+    for (long q = 0; q < n; q++) Nb[c][q] = s1_a1 * t1[c][q] + R1b[c][q];
+  }
+  interior_scaled(g, t1, 1.0, ubar);
+  adj_convdiff(g, un, Nb, ubar); /* ubar = adjoint of u^n */
+  adj_grad(g, t1, s1_b, pbar);   /* pbar = adjoint of p^n */
+  add_force_bar(g, Nb, fbar);
+
+  for (int c = 0; c < 3; c++) {
+    free(t1[c]);
+    free(t2[c]);
+    free(Sb[c]);
+    free(R1b[c]);
+    free(Nb[c]);
+  }
+  free(phib);
+  (void)interior_pt;
 }

@@ -43,6 +43,9 @@ typedef struct orc_grid {
   double one_over_2h[3], one_over_8h[3], one_over_h2Re[3], h_over_2[3],
       one_over_h[3];
   long npts[3];    /* Poisson unknowns per axis (Nx_points...)              */
+  double force[3]; /* constant body force added to the momentum right-hand
+                      side of timestep() (no reference counterpart; zero
+                      reproduces the reference)                           */
 } orc_grid;
 
 int orc_grid_init(orc_grid *g, const mif_grid *desc);
@@ -118,6 +121,16 @@ void orc_redft00(long n, const double *in, double *out);
 void orc_r2hc_direct(long n, const double *in, double *out);
 void orc_hc2r_direct(long n, const double *in, double *out);
 void orc_redft00_direct(long n, const double *in, double *out);
+
+/* Adjoint of one timestep() (no reference counterpart): given the three
+ * stencil sources the forward step read (un = u^n, y2 = Y2 after its
+ * projection, y3 = Y3 after its projection) and the adjoints of (u^{n+1},
+ * p^{n+1}) in ubar, pbar, overwrite them with the adjoints of (u^n, p^n) and
+ * add the adjoint of the body force to fbar.  Homogeneous pressure-increment
+ * conditions (timestep, not timestep_nhn); Dirichlet data are constants of
+ * the map and receive no adjoint. */
+void orc_timestep_adjoint(const orc_grid *g, double *const un[3], double *const y2[3], double *const y3[3],
+                          double *const ubar[3], double *pbar, double fbar[3]);
 
 /* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
 int orc_num_threads(void);

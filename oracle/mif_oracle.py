@@ -57,7 +57,7 @@ class OrcGrid(ctypes.Structure):
         ("N", ctypes.c_long * 3), ("base", ctypes.c_long * 3), ("h", ctypes.c_double * 3),
         ("one_over_2h", ctypes.c_double * 3), ("one_over_8h", ctypes.c_double * 3),
         ("one_over_h2Re", ctypes.c_double * 3), ("h_over_2", ctypes.c_double * 3),
-        ("one_over_h", ctypes.c_double * 3), ("npts", ctypes.c_long * 3),
+        ("one_over_h", ctypes.c_double * 3), ("npts", ctypes.c_long * 3), ("force", ctypes.c_double * 3),
     ]
 
 
@@ -119,6 +119,7 @@ def lib():
         for name in ("orc_r2hc", "orc_hc2r", "orc_redft00", "orc_r2hc_direct", "orc_hc2r_direct",
                      "orc_redft00_direct"):
             getattr(L, name).argtypes = [ctypes.c_long, dp, dp]
+        L.orc_timestep_adjoint.argtypes = [G, dpp, dpp, dpp, dpp, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -213,6 +214,20 @@ class Oracle:
     def timestep(self, vel, buf, buf2, exact_velocity, t_n, p, p_buf, grad_p=None):
         lib().orc_timestep(ctypes.byref(self.g), _dpp(vel), _dpp(buf), _dpp(buf2), ctypes.byref(exact_velocity),
                            ctypes.byref(grad_p) if grad_p is not None else None, t_n, _dp(p), _dp(p_buf))
+
+    def set_body_force(self, f):
+        """Constant body force added to the momentum right-hand side of
+        timestep() (zero reproduces the reference)."""
+        for a in range(3):
+            self.g.force[a] = float(f[a])
+
+    def timestep_adjoint(self, un, y2, y3, ubar, pbar, fbar):
+        """Adjoint of one timestep(): (ubar, pbar) hold the adjoints of
+        (u^{n+1}, p^{n+1}) on entry and of (u^n, p^n) on return; fbar (3) is
+        accumulated."""
+        assert fbar.dtype == np.float64 and fbar.shape == (3,)
+        lib().orc_timestep_adjoint(ctypes.byref(self.g), _dpp(un), _dpp(y2), _dpp(y3), _dpp(ubar), _dp(pbar),
+                                   _dp(fbar))
 
     def timestep_velocity(self, vel, buf, rhs, exact_velocity, forcing, t_n):
         lib().orc_timestep_velocity(ctypes.byref(self.g), _dpp(vel), _dpp(buf), _dpp(rhs),
