@@ -91,6 +91,85 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init):
     }
 
 
+def adjoint_leg(n: int, steps: int, k: int, directions: int, eps: float = 1e-6):
+    """Gradient workload: channel n^3, Re = 500, dt = 1e-3, `steps` steps,
+    checkpoint every k; loss 0.5 ||u(T) - u_target||^2 with u_target the
+    forward run of a seed-perturbed initial field; gradients w.r.t. u0 and a
+    constant body force.  Reports the backward/forward wall ratio (backward =
+    recompute + adjoint steps), the adjoint-step/forward-step ratio, the worst
+    relative error of the FP64 gradient against central differences over
+    seeded directions, and the FP32 adjoint's distance from the FP64 one."""
+    import numpy as np
+    import torch
+
+    import mif_amd as mif
+    from mif_amd.autograd import Rollout
+    from mif_amd.problems import adjoint_channel
+
+    gs, u0_np, ualt_np, f_np = adjoint_channel(n, steps=steps)
+    f_np = np.asarray(f_np, dtype=np.float64)
+    dev = torch.device("cuda")
+
+    def grads(precision):
+        ctx = mif.Context(gs, precision=precision, device=0)
+        roll = Rollout(ctx, steps, checkpoint_every=k)
+        target = [t.clone() for t in roll.run_forward([torch.tensor(a, device=dev) for a in ualt_np], f_np)]
+
+        def loss_of(us, ff):
+            out = roll(us, ff)
+            return 0.5 * sum(((o - t) ** 2).sum() for o, t in zip(out, target))
+
+        def run(timed):
+            u0 = [torch.tensor(a, device=dev, requires_grad=True) for a in u0_np]
+            f = torch.tensor(f_np, dtype=torch.float64, requires_grad=True)
+            roll.timed = timed
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            J = loss_of(u0, f)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            J.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            return u0, f, t1 - t0, t2 - t1
+
+        run(False)  # warm-up
+        u0, f, tf, tb = run(False)
+        _, _, _, _ = run(True)
+        split = dict(roll.last_backward)
+        return ctx, roll, loss_of, target, u0, f, tf, tb, split
+
+    ctx, roll, loss_of, target, u0, f, tf, tb, split = grads(mif.FP64)
+    gu = [t.grad.detach() for t in u0]
+    gf = f.grad.detach().numpy()
+    rng = np.random.default_rng(20261202)
+    worst = 0.0
+    with torch.no_grad():
+        for _ in range(directions):
+            du = [torch.tensor(rng.standard_normal(s), device=dev) for s in roll.shapes]
+            df = rng.standard_normal(3)
+            jp = float(loss_of([a.detach() + eps * d for a, d in zip(u0, du)], torch.tensor(f_np + eps * df)))
+            jm = float(loss_of([a.detach() - eps * d for a, d in zip(u0, du)], torch.tensor(f_np - eps * df)))
+            fd = (jp - jm) / (2 * eps)
+            ad = sum(float((g * d).sum()) for g, d in zip(gu, du)) + float(gf @ df)
+            worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-300))
+    ctx.close()
+    _, _, _, _, u0m, fm, tfm, tbm, splitm = grads(mif.MIXED)
+    num = sum(float(((a.grad - b) ** 2).sum()) for a, b in zip(u0m, gu))
+    den = sum(float((b ** 2).sum()) for b in gu)
+    return {
+        "workload": f"channel_adjoint_{n}^3", "steps": steps, "checkpoint_every": k, "Re": 500.0, "dt": 1e-3,
+        "forward_ms": tf * 1e3, "backward_ms": tb * 1e3,
+        "adjoint_over_forward": tb / tf,
+        "adjoint_step_over_forward_step": split["adjoint_s"] / max(split["recompute_s"], 1e-12),
+        "recompute_ms": split["recompute_s"] * 1e3, "adjoint_steps_ms": split["adjoint_s"] * 1e3,
+        "fd_directions": directions, "fd_eps": eps, "fd_max_rel_err": worst,
+        "fp32": {"forward_ms": tfm * 1e3, "backward_ms": tbm * 1e3, "adjoint_over_forward": tbm / tfm,
+                 "grad_u0_rel_l2_vs_fp64": (num / den) ** 0.5,
+                 "grad_f_rel_vs_fp64": float(np.linalg.norm(fm.grad.numpy() - gf) / np.linalg.norm(gf))},
+    }
+
+
 def rel_l2(a, b):
     import numpy as np
 
@@ -211,6 +290,9 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 form only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain MIB-MiB device copy (counter calibration under a profiler)")
+    ap.add_argument("--adjoint-n", type=int, default=128, help="intervals per axis of the gradient workload (0 = skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=32)
+    ap.add_argument("--adjoint-directions", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build holds the whole domain on one GPU "
@@ -258,6 +340,8 @@ def main():
     step_gbs = step_bytes * args.steps / res["seconds"] / 1e9
     cpu = cpu_baseline(args.n, args.cpu_steps) if args.cpu_steps > 0 else None
     calib = copy_calibration(args.copy_calibration) if args.copy_calibration > 0 else None
+    adj = (adjoint_leg(args.adjoint_n, args.adjoint_steps, 8, args.adjoint_directions)
+           if args.adjoint_n > 0 else None)
     line = {
         "metric": METRIC,
         "value": value,
@@ -281,6 +365,7 @@ def main():
         "step_roofline": {"alg_bytes_per_step": step_bytes, "achieved": round(step_gbs, 1), "unit": "GB/s",
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernels": table,
+        "adjoint": adj,
     }
     if calib:
         line["copy_calibration"] = calib

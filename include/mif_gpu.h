@@ -144,15 +144,26 @@ int mif_poisson_get_solution(mif_ctx *ctx, double *phi);
 /* adjust_pressure (include/PressureEquation.h:25-26,
  * src/PressureEquation.cpp:288-343) with the exact pressure at time t. */
 int mif_adjust_pressure(mif_field *pressure, const mif_function *exact_pressure, double t);
+/* The same with the exact pressure given as a table: `exact` is a node field
+ * (MIF_STAGGER_NONE) of the same context holding p_exact at every point.  For
+ * callers whose exact solution is a host callback (the reference passes a
+ * std::function, e.g. a lambda fixing the time, test/full_test.cpp:146). */
+int mif_adjust_pressure_tabulated(mif_field *pressure, const mif_field *exact);
 
 /* ---- diagnostics ------------------------------------------------------------
  * ErrorL1Norm/ErrorL2Norm/ErrorLInfNorm of the velocity (include/Norms.h:
  * 19-29, src/Norms.cpp:11-85); out = {L1, L2, Linf}. */
 int mif_error_norms_velocity(mif_field *const vel[3], const mif_function *exact_velocity, double t,
                              double out[3]);
+/* Velocity norms with the exact components given as three node fields holding
+ * the exact u, v, w at the nodes (the points src/Norms.cpp:24-44 evaluates
+ * them at); for host-callback exact solutions. */
+int mif_error_norms_velocity_tabulated(mif_field *const vel[3], mif_field *const exact_at_nodes[3], double out[3]);
 /* The scalar versions (include/Norms.h:32-34, src/Norms.cpp:88-120). */
 int mif_error_norms_pressure(const mif_field *pressure, const mif_function *exact_pressure, double t,
                              double out[3]);
+/* Scalar norms against a node-field table of the exact pressure. */
+int mif_error_norms_pressure_tabulated(const mif_field *pressure, const mif_field *exact, double out[3]);
 /* Pressure-gradient field on interior points (pressure_gradient_{u,v,w},
  * include/PressureGradient.h:9-25, as filled in test/full_test.cpp:141-142). */
 int mif_pressure_gradient(const mif_field *pressure, mif_field *const out[3]);

@@ -13,6 +13,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/mif_gpu.h"
@@ -71,10 +72,12 @@ struct mif_ctx {
   std::vector<hipEvent_t> pool;
   double total_ms[MIF_KC_COUNT]{};
   uint64_t launches[MIF_KC_COUNT]{};
+  // live fields: destroying the context frees their storage and orphans them
+  std::unordered_set<mif_field*> fields;
 };
 
 struct mif_field {
-  mif_ctx* ctx;
+  mif_ctx* ctx;  // nullptr once the owning context has been destroyed
   int stag;
   void* data;
 };
@@ -197,6 +200,7 @@ bool is_fp64(const mif_ctx* c) { return c->precision == MIF_PRECISION_FP64; }
 int check_vel(mif_field* const v[3], mif_ctx** out) {
   if (!v || !v[0] || !v[1] || !v[2]) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null velocity field");
   mif_ctx* c = v[0]->ctx;
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "field used after its context was destroyed");
   for (int q = 0; q < 3; q++) {
     if (v[q]->ctx != c) return fail(c, MIF_ERR_INVALID_ARGUMENT, "velocity components belong to different contexts");
     if (v[q]->stag != q) return fail(c, MIF_ERR_INVALID_ARGUMENT, "velocity component %d has staggering %d", q, v[q]->stag);
@@ -735,6 +739,12 @@ int mif_ctx_destroy(mif_ctx* c) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
+  for (mif_field* f : c->fields) {
+    (void)hipFree(f->data);
+    f->data = nullptr;
+    f->ctx = nullptr;
+  }
+  c->fields.clear();
   (void)hipFree(c->B);
   (void)hipFree(c->B64);
   (void)hipFree(c->P64);
@@ -801,20 +811,31 @@ int mif_field_create(mif_ctx* c, int stag, mif_field** out) {
     delete f;
     return fail(c, MIF_ERR_HIP, "field clear failed: %s", hipGetErrorString(e));
   }
+  try {
+    c->fields.insert(f);
+  } catch (...) {
+    (void)hipFree(f->data);
+    delete f;
+    return fail(c, MIF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  }
   *out = f;
   return MIF_OK;
 }
 
 int mif_field_destroy(mif_field* f) {
   if (!f) return MIF_OK;
-  (void)hipStreamSynchronize(f->ctx->stream);
-  (void)hipFree(f->data);
+  if (mif_ctx* c = f->ctx) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(f->data);
+    c->fields.erase(f);
+  }
   delete f;
   return MIF_OK;
 }
 
 int mif_field_extents(const mif_field* f, uint64_t ext[3]) {
-  if (!f || !ext) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  if (!f || !f->ctx || !ext) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
   int e[3];
   extents(f->ctx->g, f->stag, e);
   for (int a = 0; a < 3; a++) ext[a] = (uint64_t)e[a];
@@ -822,7 +843,7 @@ int mif_field_extents(const mif_field* f, uint64_t ext[3]) {
 }
 
 int mif_field_upload(mif_field* f, const double* host) {
-  if (!f || !host) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  if (!f || !f->ctx || !host) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
   mif_ctx* c = f->ctx;
   int e[3];
   extents(c->g, f->stag, e);
@@ -838,7 +859,7 @@ int mif_field_upload(mif_field* f, const double* host) {
 }
 
 int mif_field_download(const mif_field* f, double* host) {
-  if (!f || !host) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  if (!f || !f->ctx || !host) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
   mif_ctx* c = f->ctx;
   int e[3];
   extents(c->g, f->stag, e);
@@ -854,7 +875,7 @@ int mif_field_download(const mif_field* f, double* host) {
 }
 
 int mif_field_upload_device(mif_field* f, const double* dev) {
-  if (!f || !dev) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  if (!f || !f->ctx || !dev) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
   mif_ctx* c = f->ctx;
   int e[3];
   extents(c->g, f->stag, e);
@@ -868,7 +889,7 @@ int mif_field_upload_device(mif_field* f, const double* dev) {
 }
 
 int mif_field_download_device(const mif_field* f, double* dev) {
-  if (!f || !dev) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  if (!f || !f->ctx || !dev) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
   mif_ctx* c = f->ctx;
   int e[3];
   extents(c->g, f->stag, e);
@@ -882,7 +903,7 @@ int mif_field_download_device(const mif_field* f, double* dev) {
 }
 
 int mif_field_swap(mif_field* a, mif_field* b) {
-  if (!a || !b) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null field");
+  if (!a || !b || !a->ctx) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null field or destroyed context");
   if (a->ctx != b->ctx) return fail(a->ctx, MIF_ERR_INVALID_ARGUMENT, "fields belong to different contexts");
   if (a->stag != b->stag) return fail(a->ctx, MIF_ERR_INVALID_ARGUMENT, "fields have different staggering");
   std::swap(a->data, b->data);
@@ -890,7 +911,7 @@ int mif_field_swap(mif_field* a, mif_field* b) {
 }
 
 int mif_field_set(mif_field* f, const mif_function* fn, int comp, double t, int include_border) {
-  if (!f || !fn) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  if (!f || !f->ctx || !fn) return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
   mif_ctx* c = f->ctx;
   if (comp < 0 || comp > 2) return fail(c, MIF_ERR_INVALID_ARGUMENT, "component %d", comp);
   const FnDesc d = make_fn(fn);
@@ -903,7 +924,7 @@ int mif_field_set(mif_field* f, const mif_function* fn, int comp, double t, int 
 }
 
 int mif_field_apply_periodic_bc(mif_field* f) {
-  if (!f) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null field");
+  if (!f || !f->ctx) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null field");
   mif_ctx* c = f->ctx;
   if (is_fp64(c))
     launch_periodic<double>(c->g, c->L, fptr<double>(f), f->stag, c->stream);
@@ -989,7 +1010,7 @@ int mif_solve_pressure_non_homogeneous_neumann(mif_field* p, mif_field* const ve
 }
 
 int mif_poisson_solve(mif_field* p) {
-  if (!p || p->stag != MIF_STAGGER_NONE) return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad field");
+  if (!p || !p->ctx || p->stag != MIF_STAGGER_NONE) return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad field");
   mif_ctx* c = p->ctx;
   if (is_fp64(c)) {
     launch_pack<double, double>(c->g, c->L, fptr<double>(p), (double*)c->B, c->bp0, c->bplane, c->stream);
@@ -1047,17 +1068,17 @@ int mif_poisson_get_solution(mif_ctx* c, double* phi) {
   return MIF_OK;
 }
 
-int mif_adjust_pressure(mif_field* p, const mif_function* fn, double t) {
-  if (!p || !fn || p->stag != MIF_STAGGER_NONE)
-    return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad arguments");
-  mif_ctx* c = p->ctx;
-  const FnDesc d = make_fn(fn);
+namespace {
+
+// Shared bodies of adjust_pressure and the error norms: the exact solution
+// comes from a closed-form descriptor (fn) or from device tables (tab).
+int adjust_pressure_impl(mif_ctx* c, mif_field* p, const FnDesc& d, double t, const ExactTables& tab) {
   const long total = (long)c->g.npts[0] * c->g.npts[1] * c->g.npts[2];
   const int nb = reduce_blocks(total);
   if (is_fp64(c))
-    launch_norm_pressure<double>(c->g, c->L, fptr<double>(p), d, t, 1, c->partial, nb, c->red_out, c->stream);
+    launch_norm_pressure<double>(c->g, c->L, fptr<double>(p), d, t, 1, tab, c->partial, nb, c->red_out, c->stream);
   else
-    launch_norm_pressure<float>(c->g, c->L, fptr<float>(p), d, t, 1, c->partial, nb, c->red_out, c->stream);
+    launch_norm_pressure<float>(c->g, c->L, fptr<float>(p), d, t, 1, tab, c->partial, nb, c->red_out, c->stream);
   LAUNCHCHK(c);
   Red3 r;
   HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(Red3), hipMemcpyDeviceToHost, c->stream));
@@ -1072,55 +1093,101 @@ int mif_adjust_pressure(mif_field* p, const mif_function* fn, double t) {
   return MIF_OK;
 }
 
+int finish_norms(mif_ctx* c, double out[3], const char* what) {
+  Red3 r;
+  HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(Red3), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double vol = c->g.h[0] * c->g.h[1] * c->g.h[2];
+  out[0] = r.s1 * c->g.h[0] * c->g.h[1] * c->g.h[2];
+  out[1] = std::sqrt(r.s2 * vol);
+  out[2] = r.mx;
+  if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2]))
+    return fail(c, MIF_ERR_NON_FINITE, "non-finite %s error norm", what);
+  return MIF_OK;
+}
+
+int norms_velocity_impl(mif_ctx* c, mif_field* const vel[3], const FnDesc& d, double t, const ExactTables& tab,
+                        double out[3]) {
+  const long total = (long)(c->g.N[0] - 2) * (c->g.N[1] - 2) * (c->g.N[2] - 2);
+  const int nb = reduce_blocks(total);
+  if (is_fp64(c))
+    launch_norm_velocity<double>(c->g, c->L, fptr<double>(vel[0]), fptr<double>(vel[1]), fptr<double>(vel[2]), d, t,
+                                 tab, c->partial, nb, c->red_out, c->stream);
+  else
+    launch_norm_velocity<float>(c->g, c->L, fptr<float>(vel[0]), fptr<float>(vel[1]), fptr<float>(vel[2]), d, t,
+                                tab, c->partial, nb, c->red_out, c->stream);
+  LAUNCHCHK(c);
+  return finish_norms(c, out, "velocity");
+}
+
+int norms_pressure_impl(mif_ctx* c, const mif_field* p, const FnDesc& d, double t, const ExactTables& tab,
+                        double out[3]) {
+  const long total = (long)c->g.npts[0] * c->g.npts[1] * c->g.npts[2];
+  const int nb = reduce_blocks(total);
+  if (is_fp64(c))
+    launch_norm_pressure<double>(c->g, c->L, (const double*)p->data, d, t, 0, tab, c->partial, nb, c->red_out,
+                                 c->stream);
+  else
+    launch_norm_pressure<float>(c->g, c->L, (const float*)p->data, d, t, 0, tab, c->partial, nb, c->red_out,
+                                c->stream);
+  LAUNCHCHK(c);
+  return finish_norms(c, out, "pressure");
+}
+
+bool is_node_field_of(const mif_field* f, const mif_ctx* c) {
+  return f && f->ctx == c && f->stag == MIF_STAGGER_NONE;
+}
+
+}  // namespace
+
+int mif_adjust_pressure(mif_field* p, const mif_function* fn, double t) {
+  if (!p || !p->ctx || !fn || p->stag != MIF_STAGGER_NONE)
+    return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad arguments");
+  return adjust_pressure_impl(p->ctx, p, make_fn(fn), t, ExactTables{});
+}
+
+int mif_adjust_pressure_tabulated(mif_field* p, const mif_field* exact) {
+  if (!p || !p->ctx || p->stag != MIF_STAGGER_NONE || !is_node_field_of(exact, p->ctx))
+    return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad arguments");
+  ExactTables tab{};
+  tab.f[0] = exact->data;
+  return adjust_pressure_impl(p->ctx, p, FnDesc{}, 0.0, tab);
+}
+
 int mif_error_norms_velocity(mif_field* const vel[3], const mif_function* fn, double t, double out[3]) {
   mif_ctx* c = nullptr;
   int rc = check_vel(vel, &c);
   if (rc) return rc;
   if (!fn || !out) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
-  const FnDesc d = make_fn(fn);
-  const long total = (long)(c->g.N[0] - 2) * (c->g.N[1] - 2) * (c->g.N[2] - 2);
-  const int nb = reduce_blocks(total);
-  if (is_fp64(c))
-    launch_norm_velocity<double>(c->g, c->L, fptr<double>(vel[0]), fptr<double>(vel[1]), fptr<double>(vel[2]), d, t,
-                                 c->partial, nb, c->red_out, c->stream);
-  else
-    launch_norm_velocity<float>(c->g, c->L, fptr<float>(vel[0]), fptr<float>(vel[1]), fptr<float>(vel[2]), d, t,
-                                c->partial, nb, c->red_out, c->stream);
-  LAUNCHCHK(c);
-  Red3 r;
-  HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(Red3), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  const double vol = c->g.h[0] * c->g.h[1] * c->g.h[2];
-  out[0] = r.s1 * c->g.h[0] * c->g.h[1] * c->g.h[2];
-  out[1] = std::sqrt(r.s2 * vol);
-  out[2] = r.mx;
-  if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2]))
-    return fail(c, MIF_ERR_NON_FINITE, "non-finite velocity error norm");
-  return MIF_OK;
+  return norms_velocity_impl(c, vel, make_fn(fn), t, ExactTables{}, out);
+}
+
+int mif_error_norms_velocity_tabulated(mif_field* const vel[3], mif_field* const exact[3], double out[3]) {
+  mif_ctx* c = nullptr;
+  int rc = check_vel(vel, &c);
+  if (rc) return rc;
+  if (!exact || !out) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  ExactTables tab{};
+  for (int q = 0; q < 3; q++) {
+    if (!is_node_field_of(exact[q], c))
+      return fail(c, MIF_ERR_INVALID_ARGUMENT, "exact table %d is not a node field of this context", q);
+    tab.f[q] = exact[q]->data;
+  }
+  return norms_velocity_impl(c, vel, FnDesc{}, 0.0, tab, out);
 }
 
 int mif_error_norms_pressure(const mif_field* p, const mif_function* fn, double t, double out[3]) {
-  if (!p || !fn || !out || p->stag != MIF_STAGGER_NONE)
+  if (!p || !p->ctx || !fn || !out || p->stag != MIF_STAGGER_NONE)
     return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad arguments");
-  mif_ctx* c = p->ctx;
-  const FnDesc d = make_fn(fn);
-  const long total = (long)c->g.npts[0] * c->g.npts[1] * c->g.npts[2];
-  const int nb = reduce_blocks(total);
-  if (is_fp64(c))
-    launch_norm_pressure<double>(c->g, c->L, (const double*)p->data, d, t, 0, c->partial, nb, c->red_out, c->stream);
-  else
-    launch_norm_pressure<float>(c->g, c->L, (const float*)p->data, d, t, 0, c->partial, nb, c->red_out, c->stream);
-  LAUNCHCHK(c);
-  Red3 r;
-  HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(Red3), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  const double vol = c->g.h[0] * c->g.h[1] * c->g.h[2];
-  out[0] = r.s1 * c->g.h[0] * c->g.h[1] * c->g.h[2];
-  out[1] = std::sqrt(r.s2 * vol);
-  out[2] = r.mx;
-  if (!std::isfinite(out[0]) || !std::isfinite(out[1]) || !std::isfinite(out[2]))
-    return fail(c, MIF_ERR_NON_FINITE, "non-finite pressure error norm");
-  return MIF_OK;
+  return norms_pressure_impl(p->ctx, p, make_fn(fn), t, ExactTables{}, out);
+}
+
+int mif_error_norms_pressure_tabulated(const mif_field* p, const mif_field* exact, double out[3]) {
+  if (!p || !p->ctx || !out || p->stag != MIF_STAGGER_NONE || !is_node_field_of(exact, p->ctx))
+    return fail(p ? p->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad arguments");
+  ExactTables tab{};
+  tab.f[0] = exact->data;
+  return norms_pressure_impl(p->ctx, p, FnDesc{}, 0.0, tab, out);
 }
 
 int mif_pressure_gradient(const mif_field* p, mif_field* const out[3]) {
@@ -1158,7 +1225,7 @@ int mif_timestep_adjoint(mif_field* const un[3], mif_field* const y2[3], mif_fie
 }
 
 int mif_field_copy(mif_field* dst, const mif_field* src) {
-  if (!dst || !src || dst->ctx != src->ctx || dst->stag != src->stag)
+  if (!dst || !src || !dst->ctx || dst->ctx != src->ctx || dst->stag != src->stag)
     return fail(dst ? dst->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad fields");
   mif_ctx* c = dst->ctx;
   HIPCHK(c, hipMemcpyAsync(dst->data, src->data, c->L.elems * c->esize, hipMemcpyDeviceToDevice, c->stream));
@@ -1166,7 +1233,7 @@ int mif_field_copy(mif_field* dst, const mif_field* src) {
 }
 
 int mif_field_dot(const mif_field* a, const mif_field* b, double* out) {
-  if (!a || !b || !out || a->ctx != b->ctx || a->stag != b->stag)
+  if (!a || !b || !out || !a->ctx || a->ctx != b->ctx || a->stag != b->stag)
     return fail(a ? a->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad fields");
   mif_ctx* c = a->ctx;
   int e[3];

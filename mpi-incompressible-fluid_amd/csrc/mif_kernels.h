@@ -96,12 +96,18 @@ void launch_dot(const Layout& L, const int e[3], const T* a, const T* b, Red3* p
                 hipStream_t st);
 template <typename T>
 void launch_add_const(const DGrid& g, const Layout& L, T* P, double c, hipStream_t st);
+// Exact-solution tables for the norms and adjust_pressure: node-shaped
+// fields (element type of the context) holding the exact values; f[0] null
+// selects the closed-form descriptor instead.
+struct ExactTables {
+  const void* f[3];
+};
 template <typename T>
 void launch_norm_velocity(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, const FnDesc& f,
-                          double t, Red3* partial, int nblocks, Red3* out, hipStream_t st);
+                          double t, const ExactTables& tab, Red3* partial, int nblocks, Red3* out, hipStream_t st);
 template <typename T>
 void launch_norm_pressure(const DGrid& g, const Layout& L, const T* p, const FnDesc& f, double t, int adjust,
-                          Red3* partial, int nblocks, Red3* out, hipStream_t st);
+                          const ExactTables& tab, Red3* partial, int nblocks, Red3* out, hipStream_t st);
 
 // Spectral solve passes on the packed buffer B (extents npts, row pitch bp0,
 // plane pitch bplane).  dir: -1 forward, +1 inverse; axis 2 with fused = 1

@@ -500,11 +500,14 @@ __device__ inline Red3 block_reduce(Red3 v) {
 }
 
 // ErrorL1/L2/LInfNorm of the velocity (src/Norms.cpp:11-85): errors after
-// interpolating each component to the interior nodes 1..N-2.
+// interpolating each component to the interior nodes 1..N-2.  The exact
+// value comes from f, or, when tab.f[0] is not null, from three node-shaped
+// tables holding the exact components at the nodes (callers whose exact
+// solution is a host callback tabulate it).
 template <typename T>
 __global__ void __launch_bounds__(256) k_norm_velocity(DGrid g, Layout L, const T* __restrict__ u,
                                                        const T* __restrict__ v, const T* __restrict__ w, FnDesc f,
-                                                       double t, Red3* partial) {
+                                                       double t, ExactTables tab, Red3* partial) {
   const long n0 = g.N[0] - 2, n1 = g.N[1] - 2, n2 = g.N[2] - 2;
   const long total = n0 * n1 * n2;
   Red3 acc{0, 0, 0};
@@ -517,9 +520,15 @@ __global__ void __launch_bounds__(256) k_norm_velocity(DGrid g, Layout L, const 
     const double iu = ((double)u[o] + (double)u[o + 1]) / 2.0;
     const double iv = ((double)v[o] + (double)v[o + L.PX]) / 2.0;
     const double iw = ((double)w[o] + (double)w[o + L.plane]) / 2.0;
-    const double eu = fn_eval(f, 0, t, x, y, z) - iu;
-    const double ev = fn_eval(f, 1, t, x, y, z) - iv;
-    const double ew = fn_eval(f, 2, t, x, y, z) - iw;
+    double ex[3];
+    if (tab.f[0]) {
+      for (int c = 0; c < 3; c++) ex[c] = (double)((const T*)tab.f[c])[o];
+    } else {
+      for (int c = 0; c < 3; c++) ex[c] = fn_eval(f, c, t, x, y, z);
+    }
+    const double eu = ex[0] - iu;
+    const double ev = ex[1] - iv;
+    const double ew = ex[2] - iw;
     const double e2 = eu * eu + ev * ev + ew * ew;
     acc.s1 += sqrt(e2);
     acc.s2 += e2;
@@ -529,19 +538,22 @@ __global__ void __launch_bounds__(256) k_norm_velocity(DGrid g, Layout L, const 
   if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
-// Pressure error over owner points (src/Norms.cpp:88-120); with f == NULL
-// kind and mode 1 it accumulates sum(f - p) for adjust_pressure
-// (src/PressureEquation.cpp:288-295).
+// Pressure error over owner points (src/Norms.cpp:88-120); in mode 1 it
+// accumulates sum(f - p) for adjust_pressure (src/PressureEquation.cpp:
+// 288-295).  With tab.f[0] not null the exact values come from that
+// pressure-shaped table instead of f.
 template <typename T>
 __global__ void __launch_bounds__(256) k_norm_pressure(DGrid g, Layout L, const T* __restrict__ p, FnDesc f, double t,
-                                                       int adjust, Red3* partial) {
+                                                       int adjust, ExactTables tab, Red3* partial) {
   const long n0 = g.npts[0], n1 = g.npts[1], n2 = g.npts[2];
   const long total = n0 * n1 * n2;
   Red3 acc{0, 0, 0};
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
     const int i = (int)(q % n0) + g.olo[0], j = (int)((q / n0) % n1) + g.olo[1], k = (int)(q / (n0 * n1)) + g.olo[2];
     double err;
-    if (adjust) {
+    if (tab.f[0]) {
+      err = (double)((const T*)tab.f[0])[L.idx(i, j, k)] - (double)p[L.idx(i, j, k)];
+    } else if (adjust) {
       err = eval_at(g, f, 3, 0, t, i, j, k) - (double)p[L.idx(i, j, k)];
     } else {
       const double x = g.min[0] + (double)(g.base[0] + i) * g.h[0];
@@ -727,15 +739,16 @@ void launch_add_const(const DGrid& g, const Layout& L, T* P, double c, hipStream
 
 template <typename T>
 void launch_norm_velocity(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, const FnDesc& f,
-                          double t, Red3* partial, int nblocks, Red3* out, hipStream_t st) {
-  k_norm_velocity<T><<<nblocks, 256, 0, st>>>(g, L, u, v, w, f, t, partial);
+                          double t, const ExactTables& tab, Red3* partial, int nblocks, Red3* out,
+                          hipStream_t st) {
+  k_norm_velocity<T><<<nblocks, 256, 0, st>>>(g, L, u, v, w, f, t, tab, partial);
   k_reduce_final<<<1, 256, 0, st>>>(partial, nblocks, out);
 }
 
 template <typename T>
 void launch_norm_pressure(const DGrid& g, const Layout& L, const T* p, const FnDesc& f, double t, int adjust,
-                          Red3* partial, int nblocks, Red3* out, hipStream_t st) {
-  k_norm_pressure<T><<<nblocks, 256, 0, st>>>(g, L, p, f, t, adjust, partial);
+                          const ExactTables& tab, Red3* partial, int nblocks, Red3* out, hipStream_t st) {
+  k_norm_pressure<T><<<nblocks, 256, 0, st>>>(g, L, p, f, t, adjust, tab, partial);
   k_reduce_final<<<1, 256, 0, st>>>(partial, nblocks, out);
 }
 
@@ -760,9 +773,9 @@ void launch_dot(const Layout& L, const int e[3], const T* a, const T* b, Red3* p
   template void launch_dot<T>(const Layout&, const int*, const T*, const T*, Red3*, int, Red3*, hipStream_t);       \
   template void launch_add_const<T>(const DGrid&, const Layout&, T*, double, hipStream_t);                          \
   template void launch_norm_velocity<T>(const DGrid&, const Layout&, const T*, const T*, const T*, const FnDesc&,   \
-                                        double, Red3*, int, Red3*, hipStream_t);                                    \
-  template void launch_norm_pressure<T>(const DGrid&, const Layout&, const T*, const FnDesc&, double, int, Red3*,   \
-                                        int, Red3*, hipStream_t);
+                                        double, const ExactTables&, Red3*, int, Red3*, hipStream_t);                \
+  template void launch_norm_pressure<T>(const DGrid&, const Layout&, const T*, const FnDesc&, double, int,          \
+                                        const ExactTables&, Red3*, int, Red3*, hipStream_t);
 
 MIF_INSTANTIATE(double)
 MIF_INSTANTIATE(float)

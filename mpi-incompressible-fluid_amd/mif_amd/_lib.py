@@ -99,8 +99,11 @@ SIGNATURES = {
     "mif_poisson_run": [_VP],
     "mif_poisson_get_solution": [_VP, _DP],
     "mif_adjust_pressure": [_VP, _FP, _D],
+    "mif_adjust_pressure_tabulated": [_VP, _VP],
     "mif_error_norms_velocity": [_V3, _FP, _D, _DP],
+    "mif_error_norms_velocity_tabulated": [_V3, _V3, _DP],
     "mif_error_norms_pressure": [_VP, _FP, _D, _DP],
+    "mif_error_norms_pressure_tabulated": [_VP, _VP, _DP],
     "mif_pressure_gradient": [_VP, _V3],
     "mif_ctx_set_body_force": [_VP, _DP],
     "mif_timestep_adjoint": [_V3, _V3, _V3, _V3, _VP, _DP],
@@ -122,6 +125,16 @@ def load() -> ctypes.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"HIP library not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+        # One HIP runtime per process.  torch bundles its own libamdhip64
+        # (SONAME libamdhip64.so.7) and reaches it by the unversioned name
+        # through its RPATH.  When torch is loaded first, this library's
+        # libamdhip64.so.7 dependency binds to that copy; loaded the other
+        # way round the process would hold two runtimes, and torch's would
+        # see no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             fn = getattr(lib, name)

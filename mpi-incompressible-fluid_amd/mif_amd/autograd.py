@@ -15,6 +15,8 @@ the momentum right-hand side.
 """
 from __future__ import annotations
 
+import time
+
 import numpy as np
 import torch
 
@@ -41,6 +43,11 @@ class Rollout:
         self.rec = [(ctx.velocity(), ctx.velocity(), ctx.velocity()) for _ in range(min(self.k, self.nsteps))]
         self.ubar, self.pbar = ctx.velocity(), ctx.pressure()
         self.shapes = [f.shape for f in self.vel.components]
+        # seconds spent in the last backward: replay of the forward steps and
+        # the adjoint steps proper (measured only when `timed` is set, which
+        # adds a device synchronisation per checkpoint interval)
+        self.timed = False
+        self.last_backward = {"recompute_s": 0.0, "adjoint_s": 0.0}
 
     # -- device transfers with torch tensors -----------------------------------
     def _load(self, vf: VelocityField, tensors):
@@ -84,9 +91,14 @@ class Rollout:
         self._zero(self.pbar)
         fbar = np.zeros(3)
         ncp = len(self.cp_u)
+        t_rec = t_adj = 0.0
+        clock = time.perf_counter
         for c in reversed(range(ncp)):
             s0 = c * self.k
             s1 = min(self.nsteps, s0 + self.k)
+            if self.timed:
+                self.ctx.synchronize()
+                t0 = clock()
             self.vel.copy_from(self.cp_u[c])
             self.p.copy_from(self.cp_p[c])
             for s in range(s0, s1):
@@ -95,10 +107,18 @@ class Rollout:
                 self._step(s)
                 y2.copy_from(self.buf)
                 y3.copy_from(self.buf2)
+            if self.timed:
+                self.ctx.synchronize()
+                t1 = clock()
+                t_rec += t1 - t0
             for s in reversed(range(s0, s1)):
                 un, y2, y3 = self.rec[s - s0]
                 timestep_adjoint(un, y2, y3, self.ubar, self.pbar, fbar)
+            if self.timed:
+                self.ctx.synchronize()
+                t_adj += clock() - t1
         self.ctx.synchronize()
+        self.last_backward = {"recompute_s": t_rec, "adjoint_s": t_adj}
         return self._store(self.ubar), fbar
 
     def __call__(self, u0, f):

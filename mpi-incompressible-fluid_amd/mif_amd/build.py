@@ -1,9 +1,12 @@
-"""Build recipe of the HIP library (gfx950).
+"""Build recipe of the HIP library (gfx950) and of its C++ host interface.
 
 Compiles every HIP source of csrc/ to an object in parallel and links
 _build/libmif_gpu.so in-tree, so the built library travels with the source
-tree.  Invoked by __graft_entry__.build() and usable as
-`python -m mif_amd.build` from the package directory.
+tree.  Then compiles the C++ host interface of cxx/ (the reference's class
+and function signatures over the C ABI) into _build/libmif_cxx.a and links
+the C++ driver programs of tests/cpp/ against both.  Invoked by
+__graft_entry__.build() and usable as `python -m mif_amd.build` from the
+package directory.
 """
 from __future__ import annotations
 
@@ -18,6 +21,15 @@ OUT = os.path.join(PKG_DIR, "_build")
 LIB = os.path.join(OUT, "libmif_gpu.so")
 SOURCES = ["mif_api.hip", "mif_stencil.hip", "mif_fft.hip", "mif_fft_pow2.hip", "mif_mixed.hip", "mif_adjoint.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+ROOT = os.path.dirname(PKG_DIR)
+CXX_DIR = os.path.join(PKG_DIR, "cxx")
+CXX_SOURCES = ["shim.cpp", "functions.cpp"]
+CXX_LIB = os.path.join(OUT, "libmif_cxx.a")
+DRIVER_DIR = os.path.join(ROOT, "tests", "cpp")
+DRIVERS = ["full_test", "velocity_test", "pressure_test"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra",
+             "-I", os.path.join(CXX_DIR, "include"), "-I", os.path.join(ROOT, "include")]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     # no packed-FP32 instructions in generated code
@@ -44,9 +56,34 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(LIB + ".tmp", LIB)
+    build_cxx()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{r.stderr}")
+
+
+def build_cxx() -> str:
+    """libmif_cxx.a (the host interface) and the driver programs."""
+    objs = []
+    for src in CXX_SOURCES:
+        obj = os.path.join(OUT, "cxx_" + os.path.splitext(src)[0] + ".o")
+        _run([CXX, *CXX_FLAGS, "-c", os.path.join(CXX_DIR, "src", src), "-o", obj], f"{CXX} {src}")
+        objs.append(obj)
+    if os.path.exists(CXX_LIB):
+        os.remove(CXX_LIB)
+    _run(["ar", "rcs", CXX_LIB, *objs], "ar")
+    for name in DRIVERS:
+        exe = os.path.join(OUT, name)
+        _run([CXX, *CXX_FLAGS, os.path.join(DRIVER_DIR, name + ".cpp"), "-o", exe + ".tmp", CXX_LIB,
+              "-L", OUT, "-lmif_gpu", "-Wl,-rpath,$ORIGIN"], f"{CXX} {name}")
+        os.replace(exe + ".tmp", exe)
+    return CXX_LIB
 
 
 if __name__ == "__main__":

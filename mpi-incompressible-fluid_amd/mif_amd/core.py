@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import sys
+import weakref
 from dataclasses import dataclass
 from typing import Sequence
 
@@ -93,9 +94,14 @@ class Context:
         _check(self.lib.mif_ctx_info(self.ptr, ctypes.byref(dt), ext), self.ptr)
         self.dt = dt.value
         self.node_extents = tuple(int(e) for e in ext)
+        self._fields = weakref.WeakSet()
 
     def close(self):
+        """Destroy the context.  Its fields are released first; a Field
+        object that outlives the context holds no device memory."""
         if self.ptr:
+            for f in list(self._fields):
+                f.close()
             self.lib.mif_ctx_destroy(self.ptr)
             self.ptr = None
 
@@ -176,6 +182,7 @@ class Field:
         ptr = ctypes.c_void_p()
         ctx.check(ctx.lib.mif_field_create(ctx.ptr, staggering, ctypes.byref(ptr)))
         self.ptr = ptr
+        ctx._fields.add(self)
         ext = (ctypes.c_uint64 * 3)()
         ctx.check(ctx.lib.mif_field_extents(self.ptr, ext))
         self.extents = tuple(int(e) for e in ext)

@@ -95,3 +95,14 @@ def channel_initial_velocity(g: GridSpec, seed: int = 20261202, amplitude: float
     yu = mins[1] + h[1] * (base[1] + np.arange(eu[1]))
     out[0] = out[0] + (1.0 - yu ** 2)[None, :, None]
     return tuple(out)
+
+
+def adjoint_channel(n: int = 128, Re: float = 500.0, dt: float = 1e-3, steps: int = 32):
+    """The channel of the gradient workload: (grid, initial velocity, the
+    seed-perturbed initial velocity whose forward run gives the target, body
+    force).  The body force (2/Re, 0, 0) balances the viscous stress of the
+    Poiseuille profile, so the laminar part is steady."""
+    g = channel_grid(n, n, n, Re=Re, dt=dt, steps=steps)
+    u0 = channel_initial_velocity(g, seed=20261202)
+    u_alt = channel_initial_velocity(g, seed=20261203)
+    return g, u0, u_alt, (2.0 / Re, 0.0, 0.0)
