@@ -191,6 +191,8 @@ def roofline_of(res):
     table = {}
     for name, k in res["kernels"].items():
         if name not in WORDS_PER_LAUNCH:
+            # O(N^2) or auxiliary classes (boundary shell, reductions): time only
+            table[name] = {"avg_us": round(k["avg_ms"] * 1e3, 2), "launches": k["launches"]}
             continue
         bytes_per_launch = WORDS_PER_LAUNCH[name] * res["unknowns"] * res["esize"]
         gbs = bytes_per_launch / (k["avg_ms"] * 1e-3) / 1e9
@@ -317,7 +319,8 @@ def main():
     for name, r in results.items():
         rate = r["unknowns"] * args.steps / r["seconds"]
         forms[name] = {"cell_updates_per_s": rate, "ms_per_step": r["seconds"] / args.steps * 1e3,
-                       "finite": bool(all(np.isfinite(c).all() for c in r["velocity"]))}
+                       "finite": bool(all(np.isfinite(c).all() for c in r["velocity"])),
+                       "kernel_avg_us": {k: round(v["avg_ms"] * 1e3, 2) for k, v in r["kernels"].items()}}
         if name != "fp64":
             forms[name]["velocity_rel_l2_vs_fp64"] = rel_l2(r["velocity"], ref["velocity"])
             forms[name]["pressure_rel_l2_vs_fp64"] = rel_l2([r["pressure"]], [ref["pressure"]])

@@ -172,4 +172,15 @@ __device__ inline double eval_at(const DGrid& g, const FnDesc& f, int s, int c, 
   return fn_eval(f, c, t, coord(g, 0, s, i), coord(g, 1, s, j), coord(g, 2, s, k));
 }
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs of the
+// device, each with its own L2.  Mapping the dispatch index b of n
+// workgroups to this tile index gives every XCD one contiguous range of
+// tiles, so tiles that share halo lines run on the same L2.  A bijection on
+// [0, n).
+__device__ inline int xcd_tile(int b, int n) {
+  constexpr int NXCD = 8;
+  const int per = n / NXCD, rem = n % NXCD, x = b % NXCD;
+  return x * per + (x < rem ? x : rem) + b / NXCD;
+}
+
 }  // namespace mif

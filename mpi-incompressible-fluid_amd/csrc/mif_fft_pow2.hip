@@ -87,9 +87,13 @@ constexpr int ns_of(int L, int s) { return s == 0 ? 1 : ns_of(L, s - 1) * radix_
 
 // padded LDS index: one pad element every 8
 __device__ constexpr inline int pd(int e) { return e + (e >> 3); }
+// LDS pitch of one line pair: the padded length plus one element, so that
+// the lanes of a wave that address the same element of different pairs
+// (load, store and scaling phases) fall in different banks
+constexpr int lds_pitch(int L) { return L + L / 8 + 1; }
 
 template <typename T, int L, int NP, int SIGN, int S>
-__device__ inline void stage(Cx<T> (*Z)[L + L / 8], const Cx<T>* tw) {
+__device__ inline void stage(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
   constexpr int R = radix_of(L, S), Ns = ns_of(L, S), m = L / R;
   constexpr int items = NP * m;
   constexpr int per = (items + 255) / 256;
@@ -129,7 +133,7 @@ __device__ inline void stage(Cx<T> (*Z)[L + L / 8], const Cx<T>* tw) {
 }
 
 template <typename T, int L, int NP, int SIGN, int S = 0>
-__device__ inline void fft(Cx<T> (*Z)[L + L / 8], const Cx<T>* tw) {
+__device__ inline void fft(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
   if constexpr (S < nstages(L)) {
     stage<T, L, NP, SIGN, S>(Z, tw);
     fft<T, L, NP, SIGN, S + 1>(Z, tw);
@@ -152,8 +156,9 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
                                                 long bp0, long bplane, int periodic, FusedIO io) {
   // twiddles live in LDS when the workgroup stays within 64 KB, otherwise
   // they are read from the (L1-resident) global table
-  constexpr bool TWL = (NP * (L + L / 8) + L) * sizeof(Cx<T>) <= 65536;
-  __shared__ Cx<T> Z[NP][L + L / 8];
+  constexpr int LP = p2::lds_pitch(L);
+  constexpr bool TWL = (NP * LP + L) * sizeof(Cx<T>) <= 65536;
+  __shared__ Cx<T> Z[NP][LP];
   __shared__ Cx<T> tws[TWL ? L : 1];
   __shared__ long lbase[2 * NP];
   const Cx<T>* tw = TWL ? tws : (const Cx<T>*)W;
@@ -164,6 +169,19 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   const long ms = AXIS == 0 ? 1 : (AXIS == 1 ? bp0 : bplane);
   if (TWL)
     for (int e = t; e < L; e += 256) tws[e] = Cx<T>{W[2 * e], W[2 * e + 1]};
+  // eigenvalues of the scaling step (fused z pass): the z table, this
+  // workgroup's x columns and its y row, staged once
+  __shared__ double lamz[MODE == 2 ? L / 2 + 1 : 1];
+  __shared__ double lamx[MODE == 2 ? 2 * NP : 1];
+  __shared__ double lamy;
+  if (MODE == 2) {
+    for (int e = t; e <= L / 2; e += 256) lamz[e] = sc.lam[2][e];
+    if (t < NL) {
+      const int ic = blockIdx.x * NL + t;
+      lamx[t] = ic < g.npts[0] ? sc.lam[0][ic] : 0.0;
+    }
+    if (t == 0) lamy = sc.lam[1][blockIdx.y];
+  }
   if (t < NL) {
     long b = -1;
     if (AXIS == 0) {
@@ -184,36 +202,89 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
     z[l < NP ? 0 : 1] = val;
     if (!periodic && m >= 1 && m <= n - 2) ((T*)&Z[p][pd(L - m)])[l < NP ? 0 : 1] = val;
   };
+  // Global loads are issued in unrolled chunks from always-valid addresses
+  // (an element outside the lines reads index 0 and is discarded), so a
+  // chunk's loads are in flight together before its values go to LDS.
+  constexpr int R0 = (L + 255) / 256;  // elements per thread and line on axis 0
   if (AXIS == 0 && FUSED && MODE == 0) {
     const T* u = (const T*)io.u;
     const T* v = (const T*)io.v;
     const T* w = (const T*)io.w;
     const long PX = io.L.PX, PL = io.L.plane;
-    for (int l = 0; l < NL; l++) {
-      const long r = (long)blockIdx.x * NL + l;
-      const bool ok = r < (long)n1 * n2;
-      const int j = ok ? (int)(r % n1) + g.olo[1] : 0, k = ok ? (int)(r / n1) + g.olo[2] : 0;
-      for (int m = t; m < n; m += 256) {
-        T val = (T)0;
-        if (ok) {
-          const long o = io.L.idx(m + g.olo[0], j, k);
-          const double du_dx = ((double)u[o + 1] - (double)u[o]) * g.one_over_h[0];
-          const double dv_dy = ((double)v[o + PX] - (double)v[o]) * g.one_over_h[1];
-          const double dw_dz = ((double)w[o + PL] - (double)w[o]) * g.one_over_h[2];
-          val = (T)((du_dx + dv_dy + dw_dz) / io.dt);
+    constexpr int CL = R0 >= 4 ? 1 : 4 / R0;  // lines per chunk (6 loads per value)
+    for (int l0 = 0; l0 < NL; l0 += CL) {
+      T val[CL][R0];
+#pragma unroll
+      for (int a = 0; a < CL; a++) {
+        const int l = l0 + a;
+        const long r = (long)blockIdx.x * NL + l;
+        const bool okl = l < NL && r < (long)n1 * n2;
+        const int j = okl ? (int)(r % n1) + g.olo[1] : 0, k = okl ? (int)(r / n1) + g.olo[2] : 0;
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          const int m = t + 256 * q;
+          const bool okm = okl && m < n;
+          const long o = okm ? io.L.idx(m + g.olo[0], j, k) : 0;
+          const T u0 = u[o], u1 = u[o + 1], v0 = v[o], v1 = v[o + PX], w0 = w[o], w1 = w[o + PL];
+          const double du_dx = ((double)u1 - (double)u0) * g.one_over_h[0];
+          const double dv_dy = ((double)v1 - (double)v0) * g.one_over_h[1];
+          const double dw_dz = ((double)w1 - (double)w0) * g.one_over_h[2];
+          val[a][q] = okm ? (T)((du_dx + dv_dy + dw_dz) / io.dt) : (T)0;
         }
-        put(l, m, val);
       }
+#pragma unroll
+      for (int a = 0; a < CL; a++)
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          const int l = l0 + a, m = t + 256 * q;
+          if (l < NL && m < n) put(l, m, val[a][q]);
+        }
     }
   } else if (AXIS == 0) {
-    for (int l = 0; l < NL; l++) {
-      const long b = lbase[l];
-      for (int m = t; m < n; m += 256) put(l, m, b >= 0 ? B[b + m] : (T)0);
+    constexpr int CL = R0 >= 16 ? 1 : 16 / R0;  // lines per chunk: 16 values in flight
+    for (int l0 = 0; l0 < NL; l0 += CL) {
+      T val[CL][R0];
+#pragma unroll
+      for (int a = 0; a < CL; a++) {
+        const int l = l0 + a;
+        const long b = l < NL ? lbase[l] : -1;
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          const int m = t + 256 * q;
+          const bool okm = b >= 0 && m < n;
+          const T x = B[okm ? b + m : 0];
+          val[a][q] = okm ? x : (T)0;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < CL; a++)
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          const int l = l0 + a, m = t + 256 * q;
+          if (l < NL && m < n) put(l, m, val[a][q]);
+        }
     }
   } else {
     const int l = t % NL;
     const long b = lbase[l];
-    for (int m = t / NL; m < n; m += 256 / NL) put(l, m, b >= 0 ? B[b + m * ms] : (T)0);
+    constexpr int STEP = 256 / NL;        // rows per pass of the workgroup
+    constexpr int MQ = (L + STEP - 1) / STEP;  // rows per thread for n <= L
+    constexpr int CH = MQ < 16 ? MQ : 16;      // rows per chunk
+    for (int q0 = 0; q0 < MQ && (t / NL) + q0 * STEP < n; q0 += CH) {
+      T val[CH];
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int m = t / NL + (q0 + q) * STEP;
+        const bool okm = b >= 0 && m < n;
+        const T x = B[okm ? b + m * ms : 0];
+        val[q] = okm ? x : (T)0;
+      }
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int m = t / NL + (q0 + q) * STEP;
+        if (m < n) put(l, m, val[q]);
+      }
+    }
   }
   __syncthreads();
 
@@ -241,10 +312,9 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
     for (int it = t; it < NP * (L / 2 + 1); it += 256) {
       const int p = it % NP, k = it / NP;
       const int i1 = blockIdx.x * NL + p, i2 = i1 + NP;
-      const double lyj = sc.lam[1][j];
-      const double l1 = (i1 < n0 ? sc.lam[0][i1] : 0.0) + lyj, l2 = (i2 < n0 ? sc.lam[0][i2] : 0.0) + lyj;
+      const double l1 = lamx[p] + lamy, l2 = lamx[p + NP] + lamy;
       if (periodic) {
-        const double lz = sc.lam[2][k];
+        const double lz = lamz[k];
         const T s1 = (i1 == 0 && j == 0 && k == 0) ? (T)0 : (T)((1.0 / (l1 + lz)) * sc.inv_norm);
         const T s2 = (i2 == 0 && j == 0 && k == 0) ? (T)0 : (T)((1.0 / (l2 + lz)) * sc.inv_norm);
         const Cx<T> a = Z[p][pd(k)];
@@ -260,7 +330,7 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
         }
       } else {
         // DCT-I coefficients k = 0..n-1 (n = L/2 + 1), even extension kept
-        const double lz = sc.lam[2][k];
+        const double lz = lamz[k];
         const T s1 = (i1 == 0 && j == 0 && k == 0) ? (T)0 : (T)((1.0 / (l1 + lz)) * sc.inv_norm);
         const T s2 = (i2 == 0 && j == 0 && k == 0) ? (T)0 : (T)((1.0 / (l2 + lz)) * sc.inv_norm);
         const Cx<T> a = Z[p][pd(k)];

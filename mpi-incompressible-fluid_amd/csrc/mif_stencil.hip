@@ -67,6 +67,12 @@ __device__ inline bool interior(const DGrid& g, int c, int i, int j, int k) {
 // k-1, k, k+1 of the stencil source (u, v, w with a one-point halo in i and
 // j) live in an LDS ring buffer, so each source value is fetched from
 // memory about once; every tap of the 15-point stencils is an LDS read.
+// Plane k+2 is fetched into registers while plane k is computed and moved
+// to LDS afterwards, and the pointwise operands of all three components are
+// loaded together before the stencil work, so memory latency overlaps the
+// arithmetic instead of being paid once per load.  Tiles are numbered x
+// fastest, then y, then k chunk, and handed out per XCD in contiguous ranges
+// (xcd_tile), so tiles that share halo rows share an L2.
 //
 // Full step (KIND 1..3):
 //   stage 1: O1 = X + a1*N(X) - b*Gp,          O2 = N(X)        (X = u^n)
@@ -80,59 +86,113 @@ __device__ inline bool interior(const DGrid& g, int c, int i, int j, int k) {
 // stencil source is never written.
 // ---------------------------------------------------------------------------
 constexpr int STX = 64, STY = 8, SHX = STX + 2, SHY = STY + 2, SKB = 16;
+constexpr int STN = STX * STY;                  // threads per workgroup
+constexpr int PLANE_ELEMS = 3 * SHY * SHX;      // one halo plane of u, v, w
+constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;  // plane elements per thread
 
 template <typename T, int KIND>
 __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax,
-                                               int kmax) {
+                                               int kmax, int ntx, int nty, int ntiles) {
   __shared__ T pl[3][3][SHY][SHX];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * STX + tx;
-  const int i0 = blockIdx.x * STX + 1, j0 = blockIdx.y * STY + 1, k0 = blockIdx.z * SKB + 1;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
+  const int i0 = bx * STX + 1, j0 = by * STY + 1, k0 = bz * SKB + 1;
   const int k1 = min(k0 + SKB, kmax + 1);
   const int i = i0 + tx, j = j0 + ty;
   const T* X[3];
   for (int f = 0; f < 3; f++) X[f] = (const T*)(KIND <= 3 ? s.X[f] : vs.X[f]);
-  auto load_plane = [&](int k, int slot) {
-    for (int q = tid; q < 3 * SHY * SHX; q += STX * STY) {
-      const int f = q / (SHY * SHX), r = q - f * (SHY * SHX);
-      const int jj = r / SHX, ii = r - jj * SHX;
-      const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
-      T v = (T)0;
-      if (gi < L.PX && gj < L.SY && k < L.SZ) v = X[f][L.idx(gi, gj, k)];
-      pl[slot][f][jj][ii] = v;
+
+  // this thread's share of a halo plane: source column and LDS offset
+  const T* src[PQ];
+  int dst[PQ];
+  bool ok[PQ];
+#pragma unroll
+  for (int r = 0; r < PQ; r++) {
+    const int q = tid + STN * r;
+    const int f = q / (SHY * SHX), rr = q - f * (SHY * SHX);
+    const int jj = rr / SHX, ii = rr - jj * SHX;
+    const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+    ok[r] = q < PLANE_ELEMS && gi < L.PX && gj < L.SY;
+    src[r] = ok[r] ? X[f] + L.idx(gi, gj, 0) : X[0];
+    dst[r] = q < PLANE_ELEMS ? q : -1;
+  }
+  // every plane read lies in 0 .. kmax + 1 <= SZ - 1, so the load address is
+  // valid for every lane and the loads need no branch
+  auto fetch = [&](int k, T (&v)[PQ]) {
+#pragma unroll
+    for (int r = 0; r < PQ; r++) {
+      const T x = src[r][(long)k * L.plane];
+      v[r] = ok[r] ? x : (T)0;
     }
   };
-  int sm = (k0 - 1) % 3, s0 = k0 % 3, sp = (k0 + 1) % 3;
-  load_plane(k0 - 1, sm);
-  load_plane(k0, s0);
+  auto commit = [&](int slot, const T (&v)[PQ]) {
+    T* base = &pl[slot][0][0][0];
+#pragma unroll
+    for (int r = 0; r < PQ; r++)
+      if (dst[r] >= 0) base[dst[r]] = v[r];
+  };
+  {
+    T a[PQ], b[PQ], c[PQ];
+    fetch(k0 - 1, a);
+    fetch(k0, b);
+    fetch(k0 + 1, c);
+    commit((k0 - 1) % 3, a);
+    commit(k0 % 3, b);
+    commit((k0 + 1) % 3, c);
+  }
+  T pre[PQ];
+  if (k0 + 2 <= k1) fetch(k0 + 2, pre);
+  __syncthreads();
+
   const bool in_tile = i <= imax && j <= jmax;
+  const long doff[3] = {1, L.PX, L.plane};
   for (int k = k0; k < k1; k++) {
-    load_plane(k + 1, sp);
-    __syncthreads();
     if (in_tile) {
-      const int slot[3] = {sm, s0, sp};
-      auto tap = [&](int f, int di, int dj, int dk) -> T { return pl[slot[dk + 1]][f][ty + 1 + dj][tx + 1 + di]; };
       const long o = L.idx(i, j, k);
-      const long doff[3] = {1, L.PX, L.plane};
+      bool inside[3];
+      for (int c = 0; c < 3; c++) inside[c] = interior(g, c, i, j, k);
+      // pointwise operands of all three components, loaded together
+      T p0 = (T)0, pm[3] = {(T)0, (T)0, (T)0}, av[3] = {(T)0, (T)0, (T)0}, bv[3] = {(T)0, (T)0, (T)0};
+      if constexpr (KIND <= 3) {
+        const T* p = (const T*)s.p + o;
+        p0 = p[0];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          if (inside[c]) pm[c] = p[-doff[c]];
+        if constexpr (KIND >= 2) {
+#pragma unroll
+          for (int c = 0; c < 3; c++)
+            if (inside[c]) av[c] = ((const T*)s.A[c])[o];
+        }
+      } else if constexpr (KIND >= 5) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          if (inside[c]) {
+            av[c] = ((const T*)vs.R[c])[o];
+            if constexpr (KIND == 5) bv[c] = ((const T*)vs.V[c])[o];
+          }
+      }
+      const int slot[3] = {(k + 2) % 3, k % 3, (k + 1) % 3};
+      auto tap = [&](int f, int di, int dj, int dk) -> T { return pl[slot[dk + 1]][f][ty + 1 + dj][tx + 1 + di]; };
+#pragma unroll
       for (int c = 0; c < 3; c++) {
-        if (!interior(g, c, i, j, k)) continue;
+        if (!inside[c]) continue;
         T rhs = c == 0 ? rhs_u<T>(g, tap) : (c == 1 ? rhs_v<T>(g, tap) : rhs_w<T>(g, tap));
         const T x = tap(c, 0, 0, 0);
         if constexpr (KIND <= 3) {
           rhs += (T)s.force[c];
           // pressure_gradient_{u,v,w} (include/PressureGradient.h:9-25)
-          const T* p = (const T*)s.p + o;
-          const T pg = (p[0] - p[-doff[c]]) * (T)g.one_over_h[c];
+          const T pg = (p0 - pm[c]) * (T)g.one_over_h[c];
           if constexpr (KIND == 1) {
             ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs - (T)s.b * pg;
             ((T*)s.O2[c])[o] = rhs;
           } else if constexpr (KIND == 2) {
-            const T rhs_1 = ((const T*)s.A[c])[o];
             const T rhs_2_scaled = (T)s.a2 * rhs;
-            ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs_1 + rhs_2_scaled - (T)s.b * pg;
+            ((T*)s.O1[c])[o] = x + (T)s.a1 * av[c] + rhs_2_scaled - (T)s.b * pg;
             ((T*)s.O2[c])[o] = rhs_2_scaled;
           } else {
-            const T rhs_2_scaled = -((const T*)s.A[c])[o];
-            ((T*)s.O1[c])[o] = x + rhs_2_scaled + (T)s.a2 * rhs - (T)s.b * pg;
+            ((T*)s.O1[c])[o] = x + (-av[c]) + (T)s.a2 * rhs - (T)s.b * pg;
           }
         } else {
           // calculate_momentum_rhs_with_forcing_* (include/MomentumEquationForcing.h:10-31)
@@ -142,22 +202,19 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
             ((T*)vs.R[c])[o] = f;
             ((T*)vs.B[c])[o] = x + dt * (T)(8.0 / 15.0) * f;
           } else if constexpr (KIND == 5) {
-            const T r = ((T*)vs.R[c])[o];
-            const T vv = ((T*)vs.V[c])[o];
-            ((T*)vs.R[c])[o] = vv + dt * ((T)(1.0 / 4.0) * r);
-            ((T*)vs.V[c])[o] = vv + dt * ((T)(1.0 / 4.0) * r + (T)(5.0 / 12.0) * f);
+            ((T*)vs.R[c])[o] = bv[c] + dt * ((T)(1.0 / 4.0) * av[c]);
+            ((T*)vs.V[c])[o] = bv[c] + dt * ((T)(1.0 / 4.0) * av[c] + (T)(5.0 / 12.0) * f);
           } else {
-            const T r = ((T*)vs.R[c])[o];
-            ((T*)vs.B[c])[o] = r + dt * ((T)(3.0 / 4.0) * f);
+            ((T*)vs.B[c])[o] = av[c] + dt * ((T)(3.0 / 4.0) * f);
           }
         }
       }
     }
     __syncthreads();
-    const int t = sm;
-    sm = s0;
-    s0 = sp;
-    sp = t;
+    // plane k+2 replaces plane k-1; plane k+3 starts loading
+    if (k + 2 <= k1) commit((k + 2) % 3, pre);
+    if (k + 3 <= k1) fetch(k + 3, pre);
+    __syncthreads();
   }
 }
 
@@ -402,21 +459,48 @@ __global__ void __launch_bounds__(256) k_unpack(DGrid g, Layout L, const S* __re
 // 79-81) and u -= dt_s * grad(phi) on interior points
 // (update_velocity_pressure_gradient, src/Timestep.cpp:66-76).
 // ---------------------------------------------------------------------------
+// One thread per (i, j) position of the padded plane marches a chunk of k:
+// phi at k-1 stays in a register, and the operands of plane k+1 are loaded
+// while plane k is updated.  Loads read always-valid addresses and the
+// stores carry the point tests, so a plane's loads are issued together.
 template <typename T>
-__global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* u, T* v, T* w, T* p, const T* __restrict__ phi,
-                                                 double dts, int imax, int jmax, int kmax) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  if (i > imax || j > jmax || k > kmax) return;
-  const long o = L.idx(i, j, k);
-  if (i < g.N[0] && j < g.N[1] && k < g.N[2]) p[o] += phi[o];
-  const long doff[3] = {1, L.PX, L.plane};
+__global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restrict__ u, T* __restrict__ v,
+                                                 T* __restrict__ w, T* __restrict__ p, const T* __restrict__ phi,
+                                                 double dts, int nqt, int kchunk) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int qt = tile % nqt, kt = tile / nqt;
+  const int q = qt * 256 + threadIdx.x;
+  const int i = q % L.PX, j = q / L.PX;
+  if (j > g.N[1] || i > g.N[0]) return;
+  const int k0 = kt * kchunk, k1 = min(k0 + kchunk, g.N[2] + 1);
+  struct Pt {
+    T f, fx, fy, pp, a[3];
+  };
+  auto fetch = [&](int k, Pt& d) {
+    const long o = (long)q + L.plane * k;
+    d.f = phi[o];
+    d.fx = phi[o >= 1 ? o - 1 : o];
+    d.fy = phi[o >= L.PX ? o - L.PX : o];
+    d.pp = p[o];
+    d.a[0] = u[o];
+    d.a[1] = v[o];
+    d.a[2] = w[o];
+  };
+  T fz = k0 >= 1 ? phi[(long)q + L.plane * (k0 - 1)] : (T)0;
+  Pt cur;
+  fetch(k0, cur);
   T* F[3] = {u, v, w};
+  for (int k = k0; k < k1; k++) {
+    Pt nxt = cur;
+    if (k + 1 < k1) fetch(k + 1, nxt);
+    const long o = (long)q + L.plane * k;
+    if (i < g.N[0] && j < g.N[1] && k < g.N[2]) p[o] = cur.pp + cur.f;
+    const T d[3] = {cur.f - cur.fx, cur.f - cur.fy, cur.f - fz};
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    if (!interior(g, c, i, j, k)) continue;
-    F[c][o] -= (phi[o] - phi[o - doff[c]]) * (T)g.one_over_h[c] * (T)dts;
+    for (int c = 0; c < 3; c++)
+      if (interior(g, c, i, j, k)) F[c][o] = cur.a[c] - d[c] * (T)g.one_over_h[c] * (T)dts;
+    fz = cur.f;
+    cur = nxt;
   }
 }
 
@@ -605,9 +689,9 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   const int imax = max(eu[0], max(ev[0], ew[0])) - 2;
   const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
   const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
-  dim3 blk(STX, STY, 1);
-  dim3 grd((imax + STX - 1) / STX, (jmax + STY - 1) / STY, (kmax + SKB - 1) / SKB);
-  k_stage<T, KIND><<<grd, blk, 0, st>>>(g, L, s, vs, imax, jmax, kmax);
+  const int ntx = (imax + STX - 1) / STX, nty = (jmax + STY - 1) / STY, ntz = (kmax + SKB - 1) / SKB;
+  const int ntiles = ntx * nty * ntz;
+  k_stage<T, KIND><<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
 }
 
 template <typename T>
@@ -698,12 +782,13 @@ void launch_unpack(const DGrid& g, const Layout& L, const S* B, long bp0, long b
 
 template <typename T>
 void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st) {
-  const int imax = g.N[0];  // covers u's x+ ghost index too (interior test filters)
-  const int jmax = g.N[1];
-  const int kmax = g.N[2];
-  dim3 blk(64, 4, 1);
-  dim3 grd((imax + 1 + 63) / 64, (jmax + 1 + 3) / 4, kmax + 1);
-  k_project<T><<<grd, blk, 0, st>>>(g, L, u, v, w, p, phi, dts, imax, jmax, kmax);
+  // points 0..N per axis (the union of the extents; u's x+ ghost included,
+  // the interior tests filter), as 256-position slices of the padded plane
+  // times chunks of 16 planes
+  constexpr int kchunk = 16;
+  const int nqt = (int)((L.plane + 255) / 256);
+  const int nkt = (g.N[2] + 1 + kchunk - 1) / kchunk;
+  k_project<T><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk);
 }
 
 template <typename T>
