@@ -53,10 +53,9 @@ struct mif_ctx {
   double* P64 = nullptr;   // mixed form: FP64 accumulated solution
   double* R64 = nullptr;   // stand-alone Poisson runs: stored FP64 right-hand side
   double force[3]{};       // body force of timestep() (zero: the reference)
-  // adjoint workspace (allocated on first use): t1, t2, Sb, R1b, Nb triples,
-  // phib, per-block partial sums and the force-adjoint accumulator
-  void* adj[5][3]{};
-  void* adj_phib = nullptr;
+  // adjoint workspace (allocated on first use): two z_bar triples, N_bar,
+  // x_bar, per-block partial sums and the force-adjoint accumulator
+  void* adj[4][3]{};
   double* adj_partial = nullptr;
   double* adj_fbar = nullptr;
   // transfers and reductions
@@ -531,18 +530,19 @@ int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[
 //   Nb, Sb/R1b from Y; Xb = Y + J_N(X)^T Nb      (stage update transposed)
 //   pbar += b_s G^T Y;  fbar += sum Nb
 // ---------------------------------------------------------------------------
+constexpr int kAdjBlocks = 512;  // workgroups per component of k_adj_zb
+
 int adj_workspace(mif_ctx* c) {
-  if (c->adj_phib) return MIF_OK;
+  if (c->adj_partial) return MIF_OK;
   const size_t bytes = c->L.elems * c->esize;
-  for (int q = 0; q < 5; q++)
+  // z_bar of the current and of a later stage, N_bar, x_bar: zero outside
+  // the component interiors is an invariant the kernels rely on
+  for (int q = 0; q < 4; q++)
     for (int cc = 0; cc < 3; cc++) {
       HIPCHK(c, hipMalloc(&c->adj[q][cc], bytes));
       HIPCHK(c, hipMemsetAsync(c->adj[q][cc], 0, bytes, c->stream));
     }
-  HIPCHK(c, hipMalloc(&c->adj_phib, bytes));
-  HIPCHK(c, hipMemsetAsync(c->adj_phib, 0, bytes, c->stream));
-  const long blocks = (long)((c->g.N[0] + 1 + 255) / 256) * (c->g.N[1] + 1) * (c->g.N[2] + 1);
-  HIPCHK(c, hipMalloc(&c->adj_partial, 3 * blocks * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->adj_partial, 3 * kAdjBlocks * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->adj_fbar, 3 * sizeof(double)));
   return MIF_OK;
 }
@@ -550,6 +550,16 @@ int adj_workspace(mif_ctx* c) {
 AdjVec vec_of(mif_field* const f[3]) { return AdjVec{{f[0]->data, f[1]->data, f[2]->data}}; }
 AdjVec vec_ws(mif_ctx* c, int q) { return AdjVec{{c->adj[q][0], c->adj[q][1], c->adj[q][2]}}; }
 
+// Adjoint of full_timestep, stages 3, 2, 1.  Per stage s with forward
+// source X_s, step dts and pressure-gradient coefficient b_s:
+//   B     = W^-1 U^T (p_bar - dts G^T y_bar)        (k_adj_rhs)
+//   psi   = M B                                      (the forward solve)
+//   z_bar = B^T (y_bar + D^T W psi / dts)            (k_adj_zb)
+//   N_bar = stage 3: a3 z3;  stage 2: a2 (z2 - z3);  stage 1: a1 z1 + c21 z2
+//   x_bar = z_bar + J_N(X_s)^T N_bar, p_bar += b_s G^T z_bar   (k_adj_xb)
+// where y_bar is u_bar (stage 3) or the previous x_bar.  The force adjoint
+// is sum(N_bar) over the stages = (a3 - a2) S3 + (a2 + c21) S2 + a1 S1 with
+// S_s = sum(z_bar_s).
 template <typename T>
 int step_adjoint(mif_ctx* c, mif_field* const un[3], mif_field* const y2[3], mif_field* const y3[3],
                  mif_field* const ubar[3], mif_field* pbar, double fbar[3]) {
@@ -564,28 +574,29 @@ int step_adjoint(mif_ctx* c, mif_field* const un[3], mif_field* const y2[3], mif
   const double s2_a1 = -34.0 / 120.0 * dt, s2_a2 = 50.0 / 120.0 * dt, s2_b = s2_a1 + s2_a2;
   const double s3_a2 = -50.0 / 120.0 * dt, s3_a3 = 90.0 / 120.0 * dt, s3_b = s3_a2 + s3_a3;
   const double bcoef[4] = {0, s1_b, s2_b, s3_b};
-  const AdjVec T1 = vec_ws(c, 0), T2 = vec_ws(c, 1), SB = vec_ws(c, 2), R1B = vec_ws(c, 3), NB = vec_ws(c, 4);
+  // N_bar = alpha z_s + beta z_later, and the force-adjoint weight of S_s
+  const double alpha[4] = {0, s1_a1, s2_a2, s3_a3};
+  const double beta[4] = {0, s2_a1, -s2_a2, 0};
+  const double fcoef[4] = {0, s1_a1, s2_a2 + s2_a1, s3_a3 - s2_a2};
+  const AdjVec ZA = vec_ws(c, 0), ZB = vec_ws(c, 1), NB = vec_ws(c, 2), XT = vec_ws(c, 3);
   const AdjVec UB = vec_of(ubar);
   const AdjVec src[4] = {AdjVec{}, vec_of(un), vec_of(y2), vec_of(y3)};
+  // z_bar of each stage: stage 3 -> ZA, stage 2 -> ZB, stage 1 -> ZA (z3 is
+  // no longer needed); the later stage's z_bar feeding N_bar
+  const AdjVec zcur[4] = {AdjVec{}, ZA, ZB, ZA};
+  const AdjVec zlater[4] = {AdjVec{}, ZB, ZA, ZA};
   T* P = fptr<T>(pbar);
-  T* phib = (T*)c->adj_phib;
   T* B = (T*)c->B;
   HIPCHK(c, hipMemsetAsync(c->adj_fbar, 0, 3 * sizeof(double), st));
   KTimer tm(c, MIF_KC_ADJOINT);
   for (int s = 3; s >= 1; s--) {
-    // the adjoint of the velocity after this stage's projection lives in
-    // ubar (stage 3) or T2 (stages 2, 1)
-    const AdjVec Yin = s == 3 ? UB : T2;
-    launch_adj_grad<T>(g, L, P, phib, Yin, dts[s], st);
-    launch_adj_unpack<T>(g, L, phib, B, c->bp0, c->bplane, st);
+    const AdjVec Yin = s == 3 ? UB : XT;
+    launch_adj_rhs<T>(g, L, P, Yin, dts[s], B, c->bp0, c->bplane, st);
     if ((rc = spectral_solve<T>(c, B))) return rc;
-    launch_adj_div<T>(g, L, B, c->bp0, c->bplane, Yin, dts[s], st);
-    launch_adj_bc<T>(g, L, Yin, T1, st);
-    const int nblk = launch_adj_nbar<T>(s, g, L, T1, NB, SB, R1B, s == 2 ? s2_a1 : s1_a1, s2_a2, s3_a3, c->adj_partial,
-                                        st);
-    launch_sum3(c->adj_partial, nblk, c->adj_fbar, st);
-    launch_adj_convdiff<T>(g, L, src[s], NB, T1, s == 1 ? UB : T2, st);
-    launch_adj_grad<T>(g, L, P, P, T1, bcoef[s], st);
+    launch_adj_zb<T>(g, L, Yin, B, c->bp0, c->bplane, dts[s], zcur[s], NB, zlater[s], alpha[s], beta[s],
+                     c->adj_partial, kAdjBlocks, st);
+    launch_adj_sum(c->adj_partial, kAdjBlocks, fcoef[s], c->adj_fbar, st);
+    launch_adj_xb<T>(g, L, src[s], NB, zcur[s], s == 1 ? UB : XT, P, bcoef[s], st);
   }
   LAUNCHCHK(c);
   double fb[3];
@@ -749,9 +760,8 @@ int mif_ctx_destroy(mif_ctx* c) {
   (void)hipFree(c->B64);
   (void)hipFree(c->P64);
   (void)hipFree(c->R64);
-  for (int q = 0; q < 5; q++)
+  for (int q = 0; q < 4; q++)
     for (int cc = 0; cc < 3; cc++) (void)hipFree(c->adj[q][cc]);
-  (void)hipFree(c->adj_phib);
   (void)hipFree(c->adj_partial);
   (void)hipFree(c->adj_fbar);
   for (int a = 0; a < 3; a++) {

@@ -128,23 +128,22 @@ struct AdjVec {
 };
 
 // Adjoint of one step (mif_adjoint.hip).
+// Packed right-hand side of the transposed solve from p_bar and y_bar.
 template <typename T>
-void launch_adj_grad(const DGrid& g, const Layout& L, const T* in, T* out, const AdjVec& Ob, double coef,
-                     hipStream_t st);
+void launch_adj_rhs(const DGrid& g, const Layout& L, const T* P, const AdjVec& Y, double dts, T* B, long bp0,
+                    long bplane, hipStream_t st);
+// z_bar and N_bar = alpha z_bar + beta Zo of a stage, plus nblk partial sums
+// of z_bar per component (partial holds 3 * nblk doubles).
 template <typename T>
-void launch_adj_unpack(const DGrid& g, const Layout& L, const T* phib, T* B, long bp0, long bplane, hipStream_t st);
+void launch_adj_zb(const DGrid& g, const Layout& L, const AdjVec& Y, const T* Bpsi, long bp0, long bplane, double dts,
+                   const AdjVec& Zb, const AdjVec& Nb, const AdjVec& Zo, double alpha, double beta, double* partial,
+                   int nblk, hipStream_t st);
+// out[c] += coef * sum of the partials of component c.
+void launch_adj_sum(const double* partial, int nblk, double coef, double* out, hipStream_t st);
+// x_bar = z_bar + J_N(X)^T N_bar; p_bar += bcoef * G^T z_bar.
 template <typename T>
-void launch_adj_div(const DGrid& g, const Layout& L, const T* B, long bp0, long bplane, const AdjVec& yb, double dts,
-                    hipStream_t st);
-template <typename T>
-void launch_adj_bc(const DGrid& g, const Layout& L, const AdjVec& yb, const AdjVec& Yb, hipStream_t st);
-template <typename T>
-int launch_adj_nbar(int stage, const DGrid& g, const Layout& L, const AdjVec& Y, const AdjVec& Nb, const AdjVec& Sb,
-                    const AdjVec& R1b, double a1, double a2, double a3, double* partial, hipStream_t st);
-template <typename T>
-void launch_adj_convdiff(const DGrid& g, const Layout& L, const AdjVec& X, const AdjVec& Nb, const AdjVec& id,
-                         const AdjVec& Xb, hipStream_t st);
-void launch_sum3(const double* partial, long nblocks, double* out, hipStream_t st);
+void launch_adj_xb(const DGrid& g, const Layout& L, const AdjVec& X, const AdjVec& Nb, const AdjVec& Zb,
+                   const AdjVec& Xb, T* P, double bcoef, hipStream_t st);
 
 // Mixed form: FP64 right-hand side projection and defect correction
 // (mif_mixed.hip).

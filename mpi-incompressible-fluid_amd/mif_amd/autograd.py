@@ -102,11 +102,12 @@ class Rollout:
             self.vel.copy_from(self.cp_u[c])
             self.p.copy_from(self.cp_p[c])
             for s in range(s0, s1):
+                # timestep() leaves Y2 and Y3 (after their projections) in its
+                # two buffers, so the recorded triples serve as the buffers;
+                # only u^n, which the step overwrites, is copied
                 un, y2, y3 = self.rec[s - s0]
                 un.copy_from(self.vel)
-                self._step(s)
-                y2.copy_from(self.buf)
-                y3.copy_from(self.buf2)
+                timestep(self.vel, y2, y3, self.ev, s * self.ctx.dt, self.p, self.pb)
             if self.timed:
                 self.ctx.synchronize()
                 t1 = clock()
