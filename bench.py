@@ -59,22 +59,27 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init):
         mif.timestep(vel, buf, buf2, ev, t, p, pb)
         t += ctx.dt
     ctx.synchronize()
-    ctx.set_timing(True)
-    ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         mif.timestep(vel, buf, buf2, ev, t, p, pb)
         t += ctx.dt
     ctx.synchronize()
     t1 = time.perf_counter()
+    final = vel.download()
+    pres = p.download()
+    # per-kernel breakdown: a separate pass with HIP events around every
+    # launch (kept out of the timed steps above, which it would slow down)
+    ctx.set_timing(True)
+    for _ in range(max(2, min(steps, 5))):
+        mif.timestep(vel, buf, buf2, ev, t, p, pb)
+        t += ctx.dt
+    ctx.synchronize()
     kernels = {}
     for cls, name in enumerate(mif.KERNEL_CLASSES):
         ms, cnt = ctx.kernel_time(cls)
         if cnt:
             kernels[name] = {"total_ms": ms, "launches": cnt, "avg_ms": ms / cnt}
     ctx.set_timing(False)
-    final = vel.download()
-    pres = p.download()
     for f in (vel, buf, buf2):
         for comp in f.components:
             comp.close()

@@ -164,6 +164,14 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   const Cx<T>* tw = TWL ? tws : (const Cx<T>*)W;
   constexpr int NL = 2 * NP;
   const int t = threadIdx.x;
+  // tile (bx, by) from the XCD-contiguous order: neighbouring column tiles
+  // of a strided pass and neighbouring row groups of the x pass share L2
+  int bx, by;
+  {
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    bx = tile % gridDim.x;
+    by = tile / gridDim.x;
+  }
   const int n = periodic ? L : L / 2 + 1;
   const int n0 = g.npts[0], n1 = g.npts[1], n2 = g.npts[2];
   const long ms = AXIS == 0 ? 1 : (AXIS == 1 ? bp0 : bplane);
@@ -177,19 +185,19 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   if (MODE == 2) {
     for (int e = t; e <= L / 2; e += 256) lamz[e] = sc.lam[2][e];
     if (t < NL) {
-      const int ic = blockIdx.x * NL + t;
+      const int ic = bx * NL + t;
       lamx[t] = ic < g.npts[0] ? sc.lam[0][ic] : 0.0;
     }
-    if (t == 0) lamy = sc.lam[1][blockIdx.y];
+    if (t == 0) lamy = sc.lam[1][by];
   }
   if (t < NL) {
     long b = -1;
     if (AXIS == 0) {
-      const long r = (long)blockIdx.x * NL + t;
+      const long r = (long)bx * NL + t;
       if (r < (long)n1 * n2) b = bp0 * (r % n1) + bplane * (r / n1);
     } else {
-      const int i = blockIdx.x * NL + t;
-      if (i < n0) b = AXIS == 1 ? i + bplane * blockIdx.y : i + bp0 * blockIdx.y;
+      const int i = bx * NL + t;
+      if (i < n0) b = AXIS == 1 ? i + bplane * by : i + bp0 * by;
     }
     lbase[t] = b;
   }
@@ -217,7 +225,7 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
 #pragma unroll
       for (int a = 0; a < CL; a++) {
         const int l = l0 + a;
-        const long r = (long)blockIdx.x * NL + l;
+        const long r = (long)bx * NL + l;
         const bool okl = l < NL && r < (long)n1 * n2;
         const int j = okl ? (int)(r % n1) + g.olo[1] : 0, k = okl ? (int)(r / n1) + g.olo[2] : 0;
 #pragma unroll
@@ -308,10 +316,10 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
 
   if (MODE == 2) {
     // scale every mode by 1/((lx+ly+lz) * norm); constant mode -> 0
-    const int j = blockIdx.y;
+    const int j = by;
     for (int it = t; it < NP * (L / 2 + 1); it += 256) {
       const int p = it % NP, k = it / NP;
-      const int i1 = blockIdx.x * NL + p, i2 = i1 + NP;
+      const int i1 = bx * NL + p, i2 = i1 + NP;
       const double l1 = lamx[p] + lamy, l2 = lamx[p + NP] + lamy;
       if (periodic) {
         const double lz = lamz[k];
@@ -377,7 +385,7 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
       T* P = (T*)io.P;
       const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
       for (int l = 0; l < NL; l++) {
-        const long r = (long)blockIdx.x * NL + l;
+        const long r = (long)bx * NL + l;
         if (r >= (long)n1 * n2) continue;
         const int j = (int)(r % n1) + g.olo[1], k = (int)(r / n1) + g.olo[2];
         // destination rows: the owner row and its periodic images
@@ -421,7 +429,52 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   }
 }
 
-template <typename T, int L, int NP>
+// Line pairs per 256-thread workgroup, per element type, length and pass:
+// 2*NP adjacent lines, i.e. runs of 2*NP*sizeof(T) contiguous bytes on the
+// strided (y, z) passes; the LDS footprint grows with NP*L, and fewer pairs
+// per workgroup give more workgroups in flight.  The x pass reads whole
+// rows, so it takes the smallest NP that keeps 256 threads busy.
+// (Chosen by measurement on the 256^3 and 128^3 channels.)
+#ifndef MIF_NP_D64
+#define MIF_NP_D64 16
+#endif
+#ifndef MIF_NP_D128
+#define MIF_NP_D128 8
+#endif
+#ifndef MIF_NP_D256
+#define MIF_NP_D256 4
+#endif
+#ifndef MIF_NP_D512
+#define MIF_NP_D512 2
+#endif
+#ifndef MIF_NP_F64
+#define MIF_NP_F64 16
+#endif
+#ifndef MIF_NP_F128
+#define MIF_NP_F128 16
+#endif
+#ifndef MIF_NP_F256
+#define MIF_NP_F256 8
+#endif
+#ifndef MIF_NP_F512
+#define MIF_NP_F512 4
+#endif
+// x pass
+#ifndef MIF_NPX_D256
+#define MIF_NPX_D256 2
+#endif
+#ifndef MIF_NPX_D512
+#define MIF_NPX_D512 2
+#endif
+#ifndef MIF_NPX_F256
+#define MIF_NPX_F256 4
+#endif
+#ifndef MIF_NPX_F512
+#define MIF_NPX_F512 4
+#endif
+
+// PASSES: 1 = the x pass only, 2 = the y and z passes only, 3 = all.
+template <typename T, int L, int NP, int PASSES = 3>
 static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
                       long bplane, const FusedIO* io, hipStream_t st) {
   FusedIO none{};
@@ -439,21 +492,26 @@ static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, c
   }
   const int per = plan.periodic;
 #define MIF_P2(AX, MD) k_fft_p2<T, L, NP, AX, MD, false><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, none)
-  if (axis == 0 && io) {
-    if (mode == 0)
-      k_fft_p2<T, L, NP, 0, 0, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
-    else
-      k_fft_p2<T, L, NP, 0, 1, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
-  } else if (axis == 0) {
-    if (mode == 0) MIF_P2(0, 0);
-    else MIF_P2(0, 1);
-  } else if (axis == 1) {
-    if (mode == 0) MIF_P2(1, 0);
-    else MIF_P2(1, 1);
-  } else {
-    if (mode == 0) MIF_P2(2, 0);
-    else if (mode == 1) MIF_P2(2, 1);
-    else MIF_P2(2, 2);
+  if constexpr (PASSES & 1) {
+    if (axis == 0 && io) {
+      if (mode == 0)
+        k_fft_p2<T, L, NP, 0, 0, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+      else
+        k_fft_p2<T, L, NP, 0, 1, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+    } else if (axis == 0) {
+      if (mode == 0) MIF_P2(0, 0);
+      else MIF_P2(0, 1);
+    }
+  }
+  if constexpr (PASSES & 2) {
+    if (axis == 1) {
+      if (mode == 0) MIF_P2(1, 0);
+      else MIF_P2(1, 1);
+    } else if (axis == 2) {
+      if (mode == 0) MIF_P2(2, 0);
+      else if (mode == 1) MIF_P2(2, 1);
+      else MIF_P2(2, 2);
+    }
   }
 #undef MIF_P2
 }
@@ -465,16 +523,22 @@ bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, co
   constexpr bool d = sizeof(T) == 8;
   switch (plan.L) {
     case 64:
-      launch_p2<T, 64, 16>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      launch_p2<T, 64, (d ? MIF_NP_D64 : MIF_NP_F64)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 128:
-      launch_p2<T, 128, 16>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      launch_p2<T, 128, (d ? MIF_NP_D128 : MIF_NP_F128)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 256:
-      launch_p2<T, 256, (d ? 8 : 16)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      if (axis == 0)
+        launch_p2<T, 256, (d ? MIF_NPX_D256 : MIF_NPX_F256), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      else
+        launch_p2<T, 256, (d ? MIF_NP_D256 : MIF_NP_F256), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 512:
-      launch_p2<T, 512, (d ? 8 : 16)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      if (axis == 0)
+        launch_p2<T, 512, (d ? MIF_NPX_D512 : MIF_NPX_F512), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      else
+        launch_p2<T, 512, (d ? MIF_NP_D512 : MIF_NP_F512), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     default:
       return false;

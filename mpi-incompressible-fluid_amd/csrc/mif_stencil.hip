@@ -321,7 +321,8 @@ __global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
   T* F = (T*)a.F[c];
   T val;
   if (face >= 0)
-    val = (T)face_value(g, a.f, a.t, c, face, s[0], s[1], s[2]);
+    // zero data (no-slip walls): every term of face_value vanishes
+    val = a.f.kind == MIF_FN_ZERO ? (T)0 : (T)face_value(g, a.f, a.t, c, face, s[0], s[1], s[2]);
   else
     val = F[L.idx(s[0], s[1], s[2])];
   F[L.idx(i, j, k)] = val;
