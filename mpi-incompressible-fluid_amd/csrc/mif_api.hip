@@ -530,7 +530,9 @@ int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[
 //   Nb, Sb/R1b from Y; Xb = Y + J_N(X)^T Nb      (stage update transposed)
 //   pbar += b_s G^T Y;  fbar += sum Nb
 // ---------------------------------------------------------------------------
-constexpr int kAdjBlocks = 512;  // workgroups per component of k_adj_zb
+// workgroups per component of k_adj_zb: one point per thread up to this
+// many workgroups, grid-stride beyond
+constexpr int kAdjMaxBlocks = 4096;
 
 int adj_workspace(mif_ctx* c) {
   if (c->adj_partial) return MIF_OK;
@@ -542,7 +544,7 @@ int adj_workspace(mif_ctx* c) {
       HIPCHK(c, hipMalloc(&c->adj[q][cc], bytes));
       HIPCHK(c, hipMemsetAsync(c->adj[q][cc], 0, bytes, c->stream));
     }
-  HIPCHK(c, hipMalloc(&c->adj_partial, 3 * kAdjBlocks * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->adj_partial, 3 * kAdjMaxBlocks * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->adj_fbar, 3 * sizeof(double)));
   return MIF_OK;
 }
@@ -587,6 +589,8 @@ int step_adjoint(mif_ctx* c, mif_field* const un[3], mif_field* const y2[3], mif
   const AdjVec zlater[4] = {AdjVec{}, ZB, ZA, ZA};
   T* P = fptr<T>(pbar);
   T* B = (T*)c->B;
+  const long interior_pts = (long)(g.N[0] + 1) * (g.N[1] + 1) * (g.N[2] + 1);
+  const int nblk = (int)std::min<long>(kAdjMaxBlocks, (interior_pts + 255) / 256);
   HIPCHK(c, hipMemsetAsync(c->adj_fbar, 0, 3 * sizeof(double), st));
   KTimer tm(c, MIF_KC_ADJOINT);
   for (int s = 3; s >= 1; s--) {
@@ -594,8 +598,8 @@ int step_adjoint(mif_ctx* c, mif_field* const un[3], mif_field* const y2[3], mif
     launch_adj_rhs<T>(g, L, P, Yin, dts[s], B, c->bp0, c->bplane, st);
     if ((rc = spectral_solve<T>(c, B))) return rc;
     launch_adj_zb<T>(g, L, Yin, B, c->bp0, c->bplane, dts[s], zcur[s], NB, zlater[s], alpha[s], beta[s],
-                     c->adj_partial, kAdjBlocks, st);
-    launch_adj_sum(c->adj_partial, kAdjBlocks, fcoef[s], c->adj_fbar, st);
+                     c->adj_partial, nblk, st);
+    launch_adj_sum(c->adj_partial, nblk, fcoef[s], c->adj_fbar, st);
     launch_adj_xb<T>(g, L, src[s], NB, zcur[s], s == 1 ? UB : XT, P, bcoef[s], st);
   }
   LAUNCHCHK(c);
@@ -645,6 +649,10 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
   c->L.SZ = maxz;
   c->L.plane = (long)c->L.PX * c->L.SY;
   c->L.elems = c->L.plane * c->L.SZ;
+  // kernels decompose flat indices in 32 bits
+  if (c->L.elems >= (1L << 31))
+    return fail(nullptr, MIF_ERR_UNSUPPORTED, "grid too large: a field would hold %ld padded elements (limit 2^31 - 1)",
+                c->L.elems);
 
   HIPCHK(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 

@@ -272,9 +272,9 @@ __global__ void __launch_bounds__(256) k_fft(DGrid g, FftPlan P, SpectralScale s
   // element address of (line l, position m); -1 if the line is padding
   auto addr = [&](int l, int m) -> long {
     if (AXIS == 0) {
-      const long r = (long)blockIdx.x * nl + l;
-      if (r >= (long)n1 * n2) return -1;
-      const int j = (int)(r % n1), k = (int)(r / n1);
+      const int r = blockIdx.x * nl + l;
+      if (r >= n1 * n2) return -1;
+      const int j = r % n1, k = r / n1;
       return m + bp0 * j + bplane * k;
     } else {
       const int p = l % npairs, part = l / npairs;

@@ -193,8 +193,8 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   if (t < NL) {
     long b = -1;
     if (AXIS == 0) {
-      const long r = (long)bx * NL + t;
-      if (r < (long)n1 * n2) b = bp0 * (r % n1) + bplane * (r / n1);
+      const int r = bx * NL + t;
+      if (r < n1 * n2) b = bp0 * (r % n1) + bplane * (r / n1);
     } else {
       const int i = bx * NL + t;
       if (i < n0) b = AXIS == 1 ? i + bplane * by : i + bp0 * by;
@@ -225,9 +225,9 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
 #pragma unroll
       for (int a = 0; a < CL; a++) {
         const int l = l0 + a;
-        const long r = (long)bx * NL + l;
-        const bool okl = l < NL && r < (long)n1 * n2;
-        const int j = okl ? (int)(r % n1) + g.olo[1] : 0, k = okl ? (int)(r / n1) + g.olo[2] : 0;
+        const int r = bx * NL + l;
+        const bool okl = l < NL && r < n1 * n2;
+        const int j = okl ? r % n1 + g.olo[1] : 0, k = okl ? r / n1 + g.olo[2] : 0;
 #pragma unroll
         for (int q = 0; q < R0; q++) {
           const int m = t + 256 * q;
@@ -385,9 +385,9 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
       T* P = (T*)io.P;
       const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
       for (int l = 0; l < NL; l++) {
-        const long r = (long)bx * NL + l;
-        if (r >= (long)n1 * n2) continue;
-        const int j = (int)(r % n1) + g.olo[1], k = (int)(r / n1) + g.olo[2];
+        const int r = bx * NL + l;
+        if (r >= n1 * n2) continue;
+        const int j = r % n1 + g.olo[1], k = r / n1 + g.olo[2];
         // destination rows: the owner row and its periodic images
         int js[2] = {j, -1}, ks[2] = {k, -1};
         if (g.periodic[1]) js[1] = j == 1 ? N1 - 1 : (j == N1 - 2 ? 0 : -1);

@@ -25,11 +25,12 @@ __device__ inline double axis_weight(const DGrid& g, int a, int i) {
 // Partial sums of (W.b, W.1) over the packed buffer.
 __global__ void __launch_bounds__(256) k_wsum(DGrid g, const double* __restrict__ B, long bp0, long bplane,
                                               Red3* partial) {
-  const long n0 = g.npts[0], n1 = g.npts[1], n2 = g.npts[2];
-  const long total = n0 * n1 * n2;
+  const int n0 = g.npts[0], n1 = g.npts[1], n2 = g.npts[2];
+  const int total = n0 * n1 * n2;
   double s = 0, sw = 0;
-  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / (n0 * n1));
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+    const int rq = q / n0;
+    const int i = q % n0, j = rq % n1, k = rq / n1;
     const double w = axis_weight(g, 0, i) * axis_weight(g, 1, j) * axis_weight(g, 2, k);
     s += w * B[i + bp0 * j + bplane * k];
     sw += w;
