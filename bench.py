@@ -42,7 +42,7 @@ WORDS_PER_LAUNCH = {
 }
 
 
-def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init):
+def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, keep_fields: bool = True):
     import mif_amd as mif
     from mif_amd.problems import channel_grid
 
@@ -65,8 +65,16 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init):
         t += ctx.dt
     ctx.synchronize()
     t1 = time.perf_counter()
-    final = vel.download()
-    pres = p.download()
+    import numpy as np
+
+    final = vel.download() if keep_fields else None
+    pres = p.download() if keep_fields else None
+    finite = bool(all(np.isfinite(c).all() for c in final)) if keep_fields else None
+    if not keep_fields:
+        try:
+            finite = bool(np.all(np.isfinite(mif.error_norms_velocity(vel, ev, t))))
+        except mif.MifError:
+            finite = False
     # per-kernel breakdown: a separate pass with HIP events around every
     # launch (kept out of the timed steps above, which it would slow down)
     ctx.set_timing(True)
@@ -85,14 +93,18 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init):
             comp.close()
     p.close()
     pb.close()
+    solve = ctx.solve_precision()
     ctx.close()
     return {
+        "solve_transforms": "fp64" if solve[0] == mif.SOLVE_FP64 else "fp32",
+        "laplacian_condition": solve[1],
         "seconds": t1 - t0,
         "unknowns": n * (n + 1) * n,
         "esize": 8 if precision == mif.FP64 else 4,
         "kernels": kernels,
         "velocity": final,
         "pressure": pres,
+        "finite": finite,
     }
 
 
@@ -173,6 +185,29 @@ def adjoint_leg(n: int, steps: int, k: int, directions: int, eps: float = 1e-6):
                  "grad_u0_rel_l2_vs_fp64": (num / den) ** 0.5,
                  "grad_f_rel_vs_fp64": float(np.linalg.norm(fm.grad.numpy() - gf) / np.linalg.norm(gf))},
     }
+
+
+def large_leg(steps: int):
+    """The large single-GPU runs: the same channel at 768^3 in the mixed form
+    and at 512^3 in FP64 (all fields resident in HBM), cell-updates/s and
+    the per-kernel breakdown of each."""
+    import mif_amd as mif
+    from mif_amd.problems import channel_grid, channel_initial_velocity
+
+    out = {}
+    for name, n, prec in (("mixed_768^3", 768, mif.MIXED), ("fp64_512^3", 512, mif.FP64)):
+        print(f"[large] {name}: generating the initial field", file=sys.stderr, flush=True)
+        init = channel_initial_velocity(channel_grid(n, n, n))
+        print(f"[large] {name}: {steps} steps", file=sys.stderr, flush=True)
+        r = run_gpu(n, steps, 2, prec, 0, init, keep_fields=False)
+        del init
+        rate = r["unknowns"] * steps / r["seconds"]
+        roof, table = roofline_of(r)
+        out[name] = {"cell_updates_per_s": rate, "ms_per_step": r["seconds"] / steps * 1e3, "steps": steps,
+                     "grid_nodes": [n + 1] * 3, "pressure_unknowns": r["unknowns"], "finite": r["finite"],
+                     "solve_transforms": r["solve_transforms"], "laplacian_condition": r["laplacian_condition"],
+                     "kernels": table}
+    return out
 
 
 def rel_l2(a, b):
@@ -300,6 +335,8 @@ def main():
     ap.add_argument("--adjoint-n", type=int, default=128, help="intervals per axis of the gradient workload (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=32)
     ap.add_argument("--adjoint-directions", type=int, default=16)
+    ap.add_argument("--large", type=int, default=0, metavar="STEPS",
+                    help="also run the channel at 768^3 (mixed) and 512^3 (FP64) for STEPS steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build holds the whole domain on one GPU "
@@ -350,6 +387,7 @@ def main():
     calib = copy_calibration(args.copy_calibration) if args.copy_calibration > 0 else None
     adj = (adjoint_leg(args.adjoint_n, args.adjoint_steps, 8, args.adjoint_directions)
            if args.adjoint_n > 0 else None)
+    large = large_leg(args.large) if args.large > 0 else None
     line = {
         "metric": METRIC,
         "value": value,
@@ -375,6 +413,8 @@ def main():
         "kernels": table,
         "adjoint": adj,
     }
+    if large:
+        line["large"] = large
     if calib:
         line["copy_calibration"] = calib
     print(json.dumps(line), flush=True)

@@ -64,6 +64,24 @@ int mif_ctx_set_defect_sweeps(mif_ctx *ctx, int sweeps);
 /* Defect-correction sweep count of the context. */
 int mif_ctx_get_defect_sweeps(const mif_ctx *ctx, int *sweeps);
 
+/* Transforms of the mixed form's pressure solve (no reference counterpart;
+ * the reference solves in one precision).  FP32 transforms amplify their
+ * rounding by the condition number kappa of the discrete Laplacian, and
+ * both the incremental projection and the defect correction stop
+ * contracting when that amplification nears one; MIF_SOLVE_AUTO keeps FP32
+ * transforms up to kappa = 6e5 (the 512^3 channel has 4.0e5) and uses FP64
+ * transforms (FP32 fields and stencils still) above it (768^3: 8.9e5). */
+typedef enum mif_solve_precision {
+  MIF_SOLVE_AUTO = 0,
+  MIF_SOLVE_FP32 = 1,
+  MIF_SOLVE_FP64 = 2
+} mif_solve_precision;
+int mif_ctx_set_solve_precision(mif_ctx *ctx, int mode);
+/* The transforms the context will use (MIF_SOLVE_FP32 or MIF_SOLVE_FP64) and,
+ * if kappa is not NULL, the condition-number estimate max|lambda| /
+ * min|lambda != 0| of the discrete Laplacian. */
+int mif_ctx_get_solve_precision(const mif_ctx *ctx, int *effective, double *kappa);
+
 /* ---- fields ---------------------------------------------------------------
  * Replaces StaggeredTensor(const Constants&, StaggeringDirection)
  * (include/StaggeredTensor.h:32, src/StaggeredTensor.cpp:5-9); staggering is

@@ -41,6 +41,11 @@ struct mif_ctx {
   int precision = 0;
   size_t esize = 8;
   int sweeps = 1;
+  // mixed form: transforms of the pressure solve in FP32 or FP64
+  // (mif_solve_precision), and the condition number of the discrete
+  // Laplacian that the automatic choice reads
+  int solve_mode = MIF_SOLVE_AUTO;
+  double kappa = 0;
   // spectral solve
   FftPlan plan[3]{};
   double* W64[3]{};
@@ -315,6 +320,22 @@ int pressure_solve(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt,
 // the FP32 buffer c->B (sweeps == 0).  Returns which one holds it.
 enum SolutionIn { IN_B32, IN_P64 };
 
+// FP32 transforms amplify their rounding by the condition number of the
+// discrete Laplacian: an FP32 solve returns phi with an error of order
+// C eps32 kappa relative to the right-hand side's scale.  Both the
+// incremental projection (which hands each stage's pressure error to the
+// next solve) and the defect correction contract only while that factor
+// stays below one.  The 512^3 channel (kappa ~4.0e5) is stable with FP32
+// transforms and the 768^3 channel (kappa ~8.9e5) is not, so the automatic
+// choice moves the transforms to FP64 above kappa = 6e5.
+constexpr double kMaxKappaFp32 = 6.0e5;
+
+bool mixed_solve_fp64(const mif_ctx* c) {
+  if (c->solve_mode == MIF_SOLVE_FP64) return true;
+  if (c->solve_mode == MIF_SOLVE_FP32) return false;
+  return c->kappa > kMaxKappaFp32;
+}
+
 int mixed_solve_from_b64(mif_ctx* c, SolutionIn* where) {
   float* B32 = (float*)c->B;
   const long total = (long)c->g.npts[0] * c->g.npts[1] * c->g.npts[2];
@@ -376,9 +397,30 @@ int pressure_solve_mixed(mif_ctx* c, mif_field* p, mif_field* const vel[3], doub
   return MIF_OK;
 }
 
+// Mixed form with FP64 transforms: FP64 divergence of the FP32 velocity,
+// the FP64 spectral solve on the FP64 packed buffer, unpack into the FP32
+// increment.
+int pressure_solve_mixed64(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
+                           int nhn) {
+  {
+    KTimer t(c, MIF_KC_DIV);
+    launch_div_rhs<float, double>(c->g, c->L, fptr<float>(vel[0]), fptr<float>(vel[1]), fptr<float>(vel[2]), c->B64,
+                                  c->bp0, c->bplane, dt, gp, tg, nhn, c->stream);
+  }
+  int rc = spectral_solve<double>(c, c->B64);
+  if (rc) return rc;
+  {
+    KTimer t(c, MIF_KC_UNPACK);
+    launch_unpack<float, double>(c->g, c->L, c->B64, c->bp0, c->bplane, fptr<float>(p), c->stream);
+  }
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
 int pressure_solve_any(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
                        int nhn) {
   if (is_fp64(c)) return pressure_solve<double>(c, p, vel, dt, gp, tg, nhn);
+  if (mixed_solve_fp64(c)) return pressure_solve_mixed64(c, p, vel, dt, gp, tg, nhn);
   if (c->sweeps > 0 || nhn) return pressure_solve_mixed(c, p, vel, dt, gp, tg, nhn);
   return pressure_solve<float>(c, p, vel, dt, gp, tg, nhn);
 }
@@ -667,6 +709,7 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
     HIPCHK(nullptr, hipMalloc(&c->P64, c->belems * sizeof(double)));
   }
   double norm = 1.0;
+  double lam_max = 0, lam_min = 0;  // |lambda| extremes of the 3-D operator
   for (int a = 0; a < 3; a++) {
     const int n = g.npts[a];
     FftPlan& P = c->plan[a];
@@ -694,10 +737,19 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
       lam[k] = g.periodic[a] ? eig_periodic(k, g.h[a], n) : eig_neumann(k, g.h[a], n);
     HIPCHK(nullptr, hipMalloc(&c->lam[a], n * sizeof(double)));
     HIPCHK(nullptr, hipMemcpy(c->lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    double amax = 0, amin = 0;
+    for (int k = 0; k < n; k++) {
+      const double v = std::fabs(lam[k]);
+      amax = std::max(amax, v);
+      if (v > 0 && (amin == 0 || v < amin)) amin = v;
+    }
+    lam_max += amax;
+    if (amin > 0 && (lam_min == 0 || amin < lam_min)) lam_min = amin;
     // normalisation after each inverse axis (src/PressureEquation.cpp:167,200,234)
     norm *= (double)(Ng[a] - 1) * (g.periodic[a] ? 1.0 : 2.0);
   }
   c->inv_norm = 1.0 / norm;
+  c->kappa = lam_min > 0 ? lam_max / lam_min : 0;
   // transfers and reductions
   long mx = 0;
   for (int s = 0; s < 4; s++) {
@@ -803,6 +855,21 @@ int mif_ctx_set_defect_sweeps(mif_ctx* c, int sweeps) {
   if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
   if (sweeps < 0 || sweeps > 4) return fail(c, MIF_ERR_INVALID_ARGUMENT, "sweeps must be in 0..4");
   c->sweeps = sweeps;
+  return MIF_OK;
+}
+
+int mif_ctx_set_solve_precision(mif_ctx* c, int mode) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  if (mode != MIF_SOLVE_AUTO && mode != MIF_SOLVE_FP32 && mode != MIF_SOLVE_FP64)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "unknown solve precision %d", mode);
+  c->solve_mode = mode;
+  return MIF_OK;
+}
+
+int mif_ctx_get_solve_precision(const mif_ctx* c, int* effective, double* kappa) {
+  if (!c || !effective) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  *effective = (is_fp64(c) || mixed_solve_fp64(c)) ? MIF_SOLVE_FP64 : MIF_SOLVE_FP32;
+  if (kappa) *kappa = c->kappa;
   return MIF_OK;
 }
 
@@ -1063,6 +1130,10 @@ int mif_poisson_run(mif_ctx* c) {
   if (is_fp64(c)) {
     HIPCHK(c, hipMemcpyAsync(c->B, c->R64, bytes, hipMemcpyDeviceToDevice, c->stream));
     return spectral_solve<double>(c, (double*)c->B);
+  }
+  if (mixed_solve_fp64(c)) {
+    HIPCHK(c, hipMemcpyAsync(c->P64, c->R64, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return spectral_solve<double>(c, c->P64);
   }
   HIPCHK(c, hipMemcpyAsync(c->B64, c->R64, bytes, hipMemcpyDeviceToDevice, c->stream));
   SolutionIn where;

@@ -24,6 +24,9 @@ STAGGER_X, STAGGER_Y, STAGGER_Z, STAGGER_NONE = 0, 1, 2, 3
 # mif_precision
 FP64, MIXED = 0, 1
 
+# mif_solve_precision
+SOLVE_AUTO, SOLVE_FP32, SOLVE_FP64 = 0, 1, 2
+
 # mif_fn_kind
 FN_ZERO = 0
 FN_ES_VELOCITY = 1
@@ -78,6 +81,8 @@ SIGNATURES = {
     "mif_ctx_info": [_VP, _DP, _U64P],
     "mif_ctx_set_defect_sweeps": [_VP, _I],
     "mif_ctx_get_defect_sweeps": [_VP, ctypes.POINTER(_I)],
+    "mif_ctx_set_solve_precision": [_VP, _I],
+    "mif_ctx_get_solve_precision": [_VP, ctypes.POINTER(_I), _DP],
     "mif_field_create": [_VP, _I, ctypes.POINTER(_VP)],
     "mif_field_destroy": [_VP],
     "mif_field_extents": [_VP, _U64P],

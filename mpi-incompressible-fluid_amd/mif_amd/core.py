@@ -123,6 +123,18 @@ class Context:
     def set_defect_sweeps(self, n: int):
         self.check(self.lib.mif_ctx_set_defect_sweeps(self.ptr, n))
 
+    def set_solve_precision(self, mode: int):
+        """Transforms of the mixed form's pressure solve: L.SOLVE_AUTO (FP32
+        up to condition number 6e5, FP64 above), L.SOLVE_FP32, L.SOLVE_FP64."""
+        self.check(self.lib.mif_ctx_set_solve_precision(self.ptr, mode))
+
+    def solve_precision(self):
+        """(effective transform precision, condition-number estimate)."""
+        eff = ctypes.c_int()
+        kappa = ctypes.c_double()
+        self.check(self.lib.mif_ctx_get_solve_precision(self.ptr, ctypes.byref(eff), ctypes.byref(kappa)))
+        return eff.value, kappa.value
+
     def set_body_force(self, f):
         """Constant body force added to N in every stage of timestep()
         (zero = the reference's equation)."""

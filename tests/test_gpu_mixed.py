@@ -90,3 +90,52 @@ def test_mixed_timestep_velocity_error():
     got = gvel.download()
     for c in range(3):
         assert rel_err(got[c], vel[c]) < 1e-6, c
+
+
+def test_solve_precision_selection():
+    """The condition-number estimate of the discrete Laplacian and the
+    automatic FP32/FP64 choice of the mixed form's transforms."""
+    import math
+
+    from mif_amd.problems import channel_grid
+
+    n = 64
+    gs = channel_grid(n, n, n)
+    ctx = mif.Context(gs, precision=mif.MIXED)
+    eff, kappa = ctx.solve_precision()
+    # per axis: max |lambda| = 4/h^2 (k = N/2 periodic, k = n-1 Neumann);
+    # smallest non-zero |lambda| = 2(1 - cos(2 pi/N))/h^2 on a periodic axis
+    # of N points, 2(1 - cos(pi/(n-1)))/h^2 on a wall axis of n nodes
+    hx, hy, hz = 2 * math.pi / n, 2.0 / n, math.pi / n
+    lam_max = 4 / hx ** 2 + 4 / hy ** 2 + 4 / hz ** 2
+    lam_min = min(2 * (1 - math.cos(2 * math.pi / n)) / hx ** 2, 2 * (1 - math.cos(math.pi / n)) / hy ** 2,
+                  2 * (1 - math.cos(2 * math.pi / n)) / hz ** 2)
+    assert eff == mif.SOLVE_FP32
+    assert abs(kappa / (lam_max / lam_min) - 1) < 1e-9, (kappa, lam_max / lam_min)
+    ctx.set_solve_precision(mif.SOLVE_FP64)
+    assert ctx.solve_precision()[0] == mif.SOLVE_FP64
+    ctx.close()
+
+
+def test_mixed_with_fp64_transforms_tracks_fp64():
+    """FP32 fields and stencils with FP64 transforms stay within FP32
+    rounding of the FP64 form."""
+    from mif_amd.problems import channel_grid, channel_initial_velocity
+
+    gs = channel_grid(32, 32, 32, Re=1000.0, dt=5e-4, steps=5)
+    init = channel_initial_velocity(gs)
+    out = {}
+    for name, prec in (("fp64", mif.FP64), ("mixed64", mif.MIXED)):
+        ctx = mif.Context(gs, precision=prec)
+        if prec == mif.MIXED:
+            ctx.set_solve_precision(mif.SOLVE_FP64)
+        vel, buf, buf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+        p, pb = ctx.pressure(), ctx.pressure()
+        vel.upload(init)
+        for s in range(5):
+            mif.timestep(vel, buf, buf2, mif.function(mif.FN_ZERO), s * ctx.dt, p, pb)
+        out[name] = vel.download()
+        ctx.close()
+    num = sum(float(np.linalg.norm(a - b)) ** 2 for a, b in zip(out["mixed64"], out["fp64"])) ** 0.5
+    den = sum(float(np.linalg.norm(b)) ** 2 for b in out["fp64"]) ** 0.5
+    assert num / den < 1e-6, num / den
