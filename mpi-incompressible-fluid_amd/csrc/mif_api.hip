@@ -274,18 +274,16 @@ int spectral_solve(mif_ctx* c, T* B) {
   return MIF_OK;
 }
 
-bool pow2_len(int L) { return L == 64 || L == 128 || L == 256 || L == 512; }
-
 // Right-hand side from the velocity, solve, and unpack into the pressure
 // field (solve_pressure_equation_* , src/PressureEquation.cpp:266-286).  On
-// power-of-two x lengths the divergence is computed inside the x forward
+// x lengths with a specialised transform the divergence is computed inside the x forward
 // pass and the x inverse pass writes the pressure field directly.
 template <typename T>
 int pressure_solve(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
                    int nhn) {
   T* B = (T*)c->B;
   const SpectralScale sc = scale_of<T>(c);
-  if (!nhn && pow2_len(c->plan[0].L)) {
+  if (!nhn && fft_fast_len(c->plan[0].L)) {
     FusedIO io{vel[0]->data, vel[1]->data, vel[2]->data, p->data, c->L, dt};
     {
       KTimer t(c, MIF_KC_FFT_X_DIV);

@@ -1,11 +1,13 @@
-// mif_fft_pow2.hip -- transform passes for power-of-two complex lengths.
+// mif_fft_pow2.hip -- transform passes for complex lengths 2^k and 3 * 2^k.
 //
 // Same passes and slot order as mif_fft.hip (see there for the algorithm
 // and the reference it follows), specialised at compile time for
-// L = 64, 128, 256, 512 (periodic n = L, Neumann n = L/2 + 1), the lengths of
-// the 2^k-interval grids:
-//   * radix-8 Stockham stages (a final radix-4 or -2), all index arithmetic
-//     by shifts, data staged in registers so one LDS buffer suffices;
+// L = 64, 128, 256, 512, 1024 and 768, 1536 (periodic n = L, Neumann
+// n = L/2 + 1),
+// the lengths of the 2^k- and 3*2^k-interval grids:
+//   * radix-8 Stockham stages (then a radix-4 or -2 stage, and a radix-3
+//     stage for 3 * 2^k), compile-time index arithmetic, data staged in
+//     registers so one LDS buffer suffices;
 //   * twiddles in LDS; one complex pad element every 8 (the radix-8 store
 //     stride) so each group of 8 lanes of a 16-byte LDS access hits
 //     distinct banks;
@@ -55,6 +57,16 @@ __device__ inline void dft(Cx<T>* v) {
     v[2] = sub(t0, t2);
     v[1] = add(t1, t3);
     v[3] = sub(t1, t3);
+  } else if constexpr (R == 3) {
+    // w = exp(SIGN 2 pi i / 3): X1 = a - (b+c)/2 + (sqrt3/2) SIGN i (b - c)
+    const T hs = (T)0.86602540378443864676;
+    const Cx<T> a = v[0], t1 = add(v[1], v[2]);
+    const Cx<T> t2 = {a.re - (T)0.5 * t1.re, a.im - (T)0.5 * t1.im};
+    const Cx<T> d = mul_si<SIGN>(sub(v[1], v[2]));
+    const Cx<T> t3 = {hs * d.re, hs * d.im};
+    v[0] = add(a, t1);
+    v[1] = add(t2, t3);
+    v[2] = sub(t2, t3);
   } else {
     static_assert(R == 8, "radix");
     Cx<T> e[4] = {v[0], v[2], v[4], v[6]};
@@ -77,12 +89,18 @@ __device__ inline void dft(Cx<T>* v) {
 }
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
-// radix of stage s of length L: 8, 8, ..., then 4 or 2
+// L = 2^m or 3 * 2^m: stages of radix 8, 8, ..., then 4 or 2, then 3
+constexpr bool has3(int L) { return L % 3 == 0; }
+constexpr int pow2_part(int L) { return has3(L) ? L / 3 : L; }
 constexpr int radix_of(int L, int s) {
-  const int m = ilog2(L), full = m / 3;
-  return s < full ? 8 : (1 << (m % 3));
+  const int m = ilog2(pow2_part(L)), full = m / 3;
+  if (s < full) return 8;
+  if (m % 3 && s == full) return 1 << (m % 3);
+  return 3;
 }
-constexpr int nstages(int L) { return ilog2(L) / 3 + (ilog2(L) % 3 ? 1 : 0); }
+constexpr int nstages(int L) {
+  return ilog2(pow2_part(L)) / 3 + (ilog2(pow2_part(L)) % 3 ? 1 : 0) + (has3(L) ? 1 : 0);
+}
 constexpr int ns_of(int L, int s) { return s == 0 ? 1 : ns_of(L, s - 1) * radix_of(L, s - 1); }
 
 // padded LDS index: one pad element every 8
@@ -158,6 +176,8 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   // they are read from the (L1-resident) global table
   constexpr int LP = p2::lds_pitch(L);
   constexpr bool TWL = (NP * LP + L) * sizeof(Cx<T>) <= 65536;
+  static_assert(NP * LP * sizeof(Cx<T>) + (L / 2 + 1) * sizeof(double) <= 65536 - (TWL ? L * sizeof(Cx<T>) : 0),
+                "workgroup LDS above 64 KB");
   __shared__ Cx<T> Z[NP][LP];
   __shared__ Cx<T> tws[TWL ? L : 1];
   __shared__ long lbase[2 * NP];
@@ -219,7 +239,8 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
     const T* v = (const T*)io.v;
     const T* w = (const T*)io.w;
     const long PX = io.L.PX, PL = io.L.plane;
-    constexpr int CL = R0 >= 4 ? 1 : 4 / R0;  // lines per chunk (6 loads per value)
+    constexpr int CL0 = R0 >= 4 ? 1 : 4 / R0;  // lines per chunk (6 loads per value)
+    constexpr int CL = CL0 < NL ? CL0 : NL;
     for (int l0 = 0; l0 < NL; l0 += CL) {
       T val[CL][R0];
 #pragma unroll
@@ -249,7 +270,8 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
         }
     }
   } else if (AXIS == 0) {
-    constexpr int CL = R0 >= 16 ? 1 : 16 / R0;  // lines per chunk: 16 values in flight
+    constexpr int CL0 = R0 >= 16 ? 1 : 16 / R0;  // lines per chunk: 16 values in flight
+    constexpr int CL = CL0 < NL ? CL0 : NL;
     for (int l0 = 0; l0 < NL; l0 += CL) {
       T val[CL][R0];
 #pragma unroll
@@ -473,6 +495,31 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
 #define MIF_NPX_F512 4
 #endif
 
+#ifndef MIF_NP_D768
+#define MIF_NP_D768 2
+#endif
+#ifndef MIF_NP_F768
+#define MIF_NP_F768 4
+#endif
+#ifndef MIF_NP_D1024
+#define MIF_NP_D1024 1
+#endif
+#ifndef MIF_NP_F1024
+#define MIF_NP_F1024 2
+#endif
+#ifndef MIF_NP_D1536
+#define MIF_NP_D1536 1
+#endif
+#ifndef MIF_NP_F1536
+#define MIF_NP_F1536 2
+#endif
+#ifndef MIF_NPX_D768
+#define MIF_NPX_D768 1
+#endif
+#ifndef MIF_NPX_F768
+#define MIF_NPX_F768 2
+#endif
+
 // PASSES: 1 = the x pass only, 2 = the y and z passes only, 3 = all.
 template <typename T, int L, int NP, int PASSES = 3>
 static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
@@ -516,6 +563,15 @@ static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, c
 #undef MIF_P2
 }
 
+bool fft_fast_len(int L) {
+  switch (L) {
+    case 64: case 128: case 256: case 512: case 1024: case 768: case 1536:
+      return true;
+    default:
+      return false;
+  }
+}
+
 // Returns false when the plan's length has no specialised kernel.
 template <typename T>
 bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
@@ -539,6 +595,24 @@ bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, co
         launch_p2<T, 512, (d ? MIF_NPX_D512 : MIF_NPX_F512), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
         launch_p2<T, 512, (d ? MIF_NP_D512 : MIF_NP_F512), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 1024:
+      if (axis == 0)
+        launch_p2<T, 1024, 1, 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      else
+        launch_p2<T, 1024, (d ? MIF_NP_D1024 : MIF_NP_F1024), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 768:
+      if (axis == 0)
+        launch_p2<T, 768, (d ? MIF_NPX_D768 : MIF_NPX_F768), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      else
+        launch_p2<T, 768, (d ? MIF_NP_D768 : MIF_NP_F768), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 1536:
+      if (axis == 0)
+        launch_p2<T, 1536, 1, 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      else
+        launch_p2<T, 1536, (d ? MIF_NP_D1536 : MIF_NP_F1536), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     default:
       return false;

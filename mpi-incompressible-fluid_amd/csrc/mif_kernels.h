@@ -115,9 +115,11 @@ void launch_norm_pressure(const DGrid& g, const Layout& L, const T* p, const FnD
 template <typename T>
 void launch_fft_axis(const DGrid& g, int axis, int dir, int fused, const FftPlan& plan, const SpectralScale& sc, T* B,
                      long bp0, long bplane, hipStream_t st);
-// Power-of-two x pass fused with the divergence (mode 0) or the unpack into
-// the pressure field (mode 1); returns false when the length has no
-// specialised kernel (the caller then runs the separate kernels).
+// Specialised transform pass (complex lengths 2^k and 3 * 2^k, see
+// fft_fast_len); on axis 0 with io set, the forward pass is fused with the
+// divergence (mode 0) and the inverse with the unpack into the pressure field
+// (mode 1).  Returns false when the length has no specialised kernel.
+bool fft_fast_len(int L);
 template <typename T>
 bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
                      long bplane, const FusedIO* io, hipStream_t st);

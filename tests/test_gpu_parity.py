@@ -31,6 +31,16 @@ POISSON_CASES = [
     ((257, 129, 33), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),    # L = 256 (x, y)
     ((129, 257, 129), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),   # L = 128, 512, 128
     ((65, 33, 129), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),        # Neumann L = 128, 64, 256
+    # radix-3 kernels (L = 768, 1536) and L = 1024 on every axis and mode
+    ((769, 9, 17), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),      # x periodic L = 768
+    ((9, 769, 17), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),      # y Neumann L = 1536
+    ((17, 513, 9), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),      # y Neumann L = 1024
+    ((9, 9, 769), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 1)),          # z periodic L = 768 (fused z)
+    ((9, 9, 513), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),          # z Neumann L = 1024 (fused z)
+    ((5, 5, 769), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),          # z Neumann L = 1536 (fused z)
+    ((513, 9, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),          # x Neumann L = 1024
+    ((769, 5, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),          # x Neumann L = 1536
+    ((9, 769, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 1, 0)),          # y periodic L = 768
 ]
 
 
@@ -164,6 +174,16 @@ def test_full_timestep_channel_pow2_fused_path():
     _print_case("timestep channel 65x33x65")
     o, vel, p, gvel, gp = _run_full((65, 33, 65), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 5e-3, 3, (1, 0, 1),
                                     "channel", mo.FN_ZERO, seed=5)
+    for c in range(3):
+        assert rel_err(gvel[c], vel[c]) < TOL, c
+    assert rel_err(gp, p) < TOL
+
+
+def test_full_timestep_channel_radix3_fused_path():
+    # x length 768 = 3 * 2^8: the fused x passes of the radix-3 kernels
+    _print_case("timestep channel 769x9x17")
+    o, vel, p, gvel, gp = _run_full((769, 9, 17), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 1e-3, 2, (1, 0, 1),
+                                    "channel", mo.FN_ZERO, seed=7)
     for c in range(3):
         assert rel_err(gvel[c], vel[c]) < TOL, c
     assert rel_err(gp, p) < TOL
