@@ -245,21 +245,21 @@ def roofline_of(res):
     return roof, table
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_channel_256cubed_fp64.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_channel_256cubed.md")
 CLASS_KERNELS = {
-    "stage": r"k_stage<double, [123]>",
-    "fft_x_div": r"k_fft_p2<double, \d+, \d+, 0, 0, true>",
-    "fft_x_unpack": r"k_fft_p2<double, \d+, \d+, 0, 1, true>",
-    "fft_y": r"k_fft_p2<double, \d+, \d+, 1, [01], false>",
-    "fft_z_fused": r"k_fft_p2<double, \d+, \d+, 2, 2, false>",
-    "project": r"k_project<double>",
+    "stage": r"k_stage<{T}, [123]>",
+    "fft_x_div": r"k_fft_p2<{T}, \d+, \d+, 0, 0, true>",
+    "fft_x_unpack": r"k_fft_p2<{T}, \d+, \d+, 0, 1, true>",
+    "fft_y": r"k_fft_p2<{T}, \d+, \d+, 1, [01], false>",
+    "fft_z_fused": r"k_fft_p2<{T}, \d+, \d+, 2, 2, false>",
+    "project": r"k_project<{T}>",
 }
 
 
-def measured_traffic(cls: str):
-    """HBM bytes per launch of a kernel class at the 256^3 FP64 workload, from
-    the committed PMC summary (FETCH_SIZE and WRITE_SIZE scaled by the plain
-    copy calibration)."""
+def measured_traffic(cls: str, esize: int):
+    """HBM bytes per launch of a kernel class at the 256^3 workload (FP64 or
+    FP32 kernels by esize), from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE scaled by the plain copy calibration)."""
     import re
 
     if cls not in CLASS_KERNELS or not os.path.exists(TRAFFIC_PROFILE):
@@ -268,7 +268,8 @@ def measured_traffic(cls: str):
     if not last:
         return None
     data = json.loads(last[-1])["kernels"]
-    vals = [v["total_bytes"] for k, v in data.items() if re.search(CLASS_KERNELS[cls], k)]
+    pat = CLASS_KERNELS[cls].replace("{T}", "double" if esize == 8 else "float")
+    vals = [v["total_bytes"] for k, v in data.items() if re.search(pat, k)]
     return round(sum(vals) / len(vals)) if vals else None
 
 
@@ -356,6 +357,7 @@ def main():
         runs["mixed_s1"] = (mif.MIXED, 1)
     results = {}
     for name, (prec, sweeps) in runs.items():
+        print(f"[bench] {name}: {args.warmup} + {args.steps} steps at {args.n}^3", file=sys.stderr, flush=True)
         results[name] = run_gpu(args.n, args.steps, args.warmup, prec, sweeps, init)
     ref = results["fp64"]
     for name, r in results.items():
@@ -378,13 +380,17 @@ def main():
     res = results[chosen]
     value = forms[chosen]["cell_updates_per_s"]
     roof, table = roofline_of(res)
-    if roof is not None and chosen == "fp64" and args.n == 256:
-        roof["traffic"] = measured_traffic(roof["kernel"])
+    if roof is not None and args.n == 256:
+        roof["traffic"] = measured_traffic(roof["kernel"], res["esize"])
     step_words = 108
     step_bytes = step_words * res["unknowns"] * res["esize"]
     step_gbs = step_bytes * args.steps / res["seconds"] / 1e9
+    if args.cpu_steps > 0:
+        print(f"[bench] CPU baseline: {args.cpu_steps} steps", file=sys.stderr, flush=True)
     cpu = cpu_baseline(args.n, args.cpu_steps) if args.cpu_steps > 0 else None
     calib = copy_calibration(args.copy_calibration) if args.copy_calibration > 0 else None
+    if args.adjoint_n > 0:
+        print(f"[bench] adjoint: {args.adjoint_n}^3, {args.adjoint_steps} steps", file=sys.stderr, flush=True)
     adj = (adjoint_leg(args.adjoint_n, args.adjoint_steps, 8, args.adjoint_directions)
            if args.adjoint_n > 0 else None)
     large = large_leg(args.large) if args.large > 0 else None

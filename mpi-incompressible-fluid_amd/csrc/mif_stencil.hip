@@ -146,6 +146,7 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
   __syncthreads();
 
   const bool in_tile = i <= imax && j <= jmax;
+
   const long doff[3] = {1, L.PX, L.plane};
   for (int k = k0; k < k1; k++) {
     if (in_tile) {
@@ -266,7 +267,9 @@ struct BCArgs {
   int count[3][3];  // number of shell points per (component, face pair)
 };
 
-template <typename T>
+// ZERO: homogeneous data (no-slip walls), every face value is 0; the
+// function evaluation code is compiled out, which keeps the kernel small.
+template <typename T, bool ZERO>
 __global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
   const int c = blockIdx.y / 3, pair = blockIdx.y % 3;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,10 +323,12 @@ __global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
   }
   T* F = (T*)a.F[c];
   T val;
-  if (face >= 0)
-    // zero data (no-slip walls): every term of face_value vanishes
-    val = a.f.kind == MIF_FN_ZERO ? (T)0 : (T)face_value(g, a.f, a.t, c, face, s[0], s[1], s[2]);
-  else
+  if (face >= 0) {
+    if constexpr (ZERO)
+      val = (T)0;
+    else
+      val = (T)face_value(g, a.f, a.t, c, face, s[0], s[1], s[2]);
+  } else
     val = F[L.idx(s[0], s[1], s[2])];
   F[L.idx(i, j, k)] = val;
 }
@@ -737,7 +742,10 @@ void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& 
   a.f = f;
   a.t = t;
   dim3 grd((mx + 255) / 256, 9, 1);
-  k_bc<T><<<grd, 256, 0, st>>>(g, L, a);
+  if (f.kind == MIF_FN_ZERO)
+    k_bc<T, true><<<grd, 256, 0, st>>>(g, L, a);
+  else
+    k_bc<T, false><<<grd, 256, 0, st>>>(g, L, a);
 }
 
 template <typename T>

@@ -3,13 +3,14 @@ profiles/.
 
   python -m mif_amd.rocprof_summary stats  <kernel_stats.csv>
   python -m mif_amd.rocprof_summary traffic <fetch counter_collection.csv> \
-        <write counter_collection.csv> --copy-mib 1024 --unknowns N --esize 8
+        <write counter_collection.csv> --copy-mib 1024 --unknowns N
 
 `traffic` scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch) by what the
 same counters read for the plain device copy of --copy-mib MiB that
 `bench.py --copy-calibration` performs, and prints per kernel the measured
 bytes per launch next to the algorithmic bytes (words per pressure unknown
-x unknowns x element size).  The last line is a JSON object with the same
+x unknowns x element size; the element size is read from the kernel's
+template type, float or double, unless --esize is given).  The last line is a JSON object with the same
 figures, which bench.py reads to fill the roofline's `traffic` field.
 """
 from __future__ import annotations
@@ -62,6 +63,11 @@ def _counter(path: str, ctr: str):
     return d
 
 
+def element_size(name: str) -> int:
+    m = re.search(r"<(float|double)", name)
+    return 4 if m and m.group(1) == "float" else 8
+
+
 def traffic(fetch_csv: str, write_csv: str, copy_mib: int, unknowns: int, esize: int):
     f = _counter(fetch_csv, "FETCH_SIZE")
     w = _counter(write_csv, "WRITE_SIZE")
@@ -84,7 +90,7 @@ def traffic(fetch_csv: str, write_csv: str, copy_mib: int, unknowns: int, esize:
             continue
         rd = sum(f[name]) / len(f[name]) * sf * 1024
         wr = sum(w.get(name, [0])) / max(1, len(w.get(name, []))) * sw * 1024
-        alg = words * unknowns * esize
+        alg = words * unknowns * (esize or element_size(name))
         key = short(name)
         out[key] = {"read_bytes": rd, "write_bytes": wr, "total_bytes": rd + wr, "alg_bytes": alg}
         print(f"| `{key}` | {len(f[name])} | {rd / 1e6:.1f} | {wr / 1e6:.1f} | {(rd + wr) / 1e6:.1f} | "
@@ -103,7 +109,7 @@ def main():
     t.add_argument("write_csv")
     t.add_argument("--copy-mib", type=int, required=True)
     t.add_argument("--unknowns", type=int, required=True)
-    t.add_argument("--esize", type=int, default=8)
+    t.add_argument("--esize", type=int, default=0, help="element size in bytes (default: from the kernel type)")
     a = ap.parse_args()
     if a.cmd == "stats":
         stats(a.csv)
