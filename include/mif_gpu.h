@@ -195,10 +195,16 @@ int mif_ctx_set_body_force(mif_ctx *ctx, const double force[3]);
  * sources: u^n, and the `buf` and `buf2` triples as mif_timestep leaves them
  * (Y2 and Y3 after their projections).  On entry ubar and pbar hold the
  * adjoints of (u^{n+1}, p^{n+1}); on return those of (u^n, p^n).
- * force_bar[3] is incremented by the adjoint of the body force.  Dirichlet
- * data are constants of the map.  Synchronises the stream. */
+ * force_bar[3] is incremented by the adjoint of the body force, which
+ * synchronises the stream; with force_bar == NULL it is added to an
+ * accumulator in the context instead and the call does not synchronise.
+ * Dirichlet data are constants of the map. */
 int mif_timestep_adjoint(mif_field *const un[3], mif_field *const y2[3], mif_field *const y3[3],
                          mif_field *const ubar[3], mif_field *pbar, double force_bar[3]);
+/* Adds the context's force-adjoint accumulator (the calls above with
+ * force_bar == NULL since the last take) to force_bar[3] and resets it.
+ * Synchronises the stream. */
+int mif_adjoint_force_take(mif_ctx *ctx, double force_bar[3]);
 /* Device copy between fields of one staggering (Tensor copy). */
 int mif_field_copy(mif_field *dst, const mif_field *src);
 /* Sum over all points of the extents of a * b (FP64 accumulation). */

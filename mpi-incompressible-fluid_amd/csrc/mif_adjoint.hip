@@ -123,13 +123,15 @@ __global__ void __launch_bounds__(256) k_adj_rhs(DGrid g, Layout L, const T* __r
 //   z_bar_c(Q) = sum over the points g holding Q (Q and its periodic ghost
 //                copies) of y_bar_c(g) + (Wpsi(g - e_c) - Wpsi(g)) / (dt_s h_c)
 //   N_bar_c(Q) = alpha z_bar_c(Q) + beta Zo_c(Q)
-// with Wpsi = W * psi on owner nodes (zero elsewhere), and one partial sum
-// of z_bar_c per workgroup.
+// with Wpsi = W * psi on owner nodes (zero elsewhere), and per workgroup the
+// partial sum of z_bar_c weighted by fcoef, stored (accumulate = 0) or added
+// to the workgroup's slot (accumulate = 1: the later stages of one adjoint
+// step, launched with the same grid, so slot b always belongs to workgroup b).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_adj_zb(DGrid g, Layout L, AdjVec Y, const T* __restrict__ Bpsi, long bp0,
                                                 long bplane, double dts, AdjVec Zb, AdjVec Nb, AdjVec Zo, double alpha,
-                                                double beta, double* __restrict__ partial) {
+                                                double beta, double fcoef, int accumulate, double* __restrict__ partial) {
   const int c = blockIdx.y;
   int e[3];
   extents(g, c, e);
@@ -184,25 +186,42 @@ __global__ void __launch_bounds__(256) k_adj_zb(DGrid g, Layout L, AdjVec Y, con
   for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = part;
   __syncthreads();
-  if (threadIdx.x == 0) partial[c * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+  if (threadIdx.x == 0) {
+    const double v = fcoef * (sh[0] + sh[1] + sh[2] + sh[3]);
+    double* slot = &partial[c * gridDim.x + blockIdx.x];
+    *slot = accumulate ? *slot + v : v;
+  }
 }
 
-// out[c] += coef * (sum of the nblk partials of component c), one workgroup,
-// fixed summation order.
-__global__ void __launch_bounds__(256) k_adj_sum(const double* __restrict__ partial, int nblk, double coef,
-                                                 double* __restrict__ out) {
-  __shared__ double sh[256];
-  for (int c = 0; c < 3; c++) {
-    double s = 0;
-    for (int q = threadIdx.x; q < nblk; q += blockDim.x) s += partial[c * nblk + q];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int stp = blockDim.x / 2; stp > 0; stp >>= 1) {
-      if (threadIdx.x < stp) sh[threadIdx.x] += sh[threadIdx.x + stp];
-      __syncthreads();
+// out[c] (= or +=) sum of the nblk partials of component c: one workgroup of
+// 1024 threads, each loading its kAdjSumPer values per component at once,
+// then a fixed-order wave and workgroup reduction (deterministic).
+constexpr int kAdjSumThreads = 1024, kAdjSumPer = 4;
+static_assert(kAdjSumThreads * kAdjSumPer >= kAdjMaxBlocks, "k_adj_sum covers every partial");
+__global__ void __launch_bounds__(kAdjSumThreads) k_adj_sum(const double* __restrict__ partial, int nblk,
+                                                            int accumulate, double* __restrict__ out) {
+  double v[3][kAdjSumPer];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int q = 0; q < kAdjSumPer; q++) {
+      const int b = threadIdx.x + kAdjSumThreads * q;
+      const double x = partial[c * nblk + (b < nblk ? b : 0)];
+      v[c][q] = b < nblk ? x : 0.0;
     }
-    if (threadIdx.x == 0) out[c] += coef * sh[0];
-    __syncthreads();
+  __shared__ double sh[3][kAdjSumThreads / 64];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double s = (v[c][0] + v[c][1]) + (v[c][2] + v[c][3]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int c = threadIdx.x;
+    double s = 0;
+    for (int w = 0; w < kAdjSumThreads / 64; w++) s += sh[c][w];
+    out[c] = accumulate ? out[c] + s : s;
   }
 }
 
@@ -466,13 +485,15 @@ void launch_adj_rhs(const DGrid& g, const Layout& L, const T* P, const AdjVec& Y
 
 template <typename T>
 void launch_adj_zb(const DGrid& g, const Layout& L, const AdjVec& Y, const T* Bpsi, long bp0, long bplane, double dts,
-                   const AdjVec& Zb, const AdjVec& Nb, const AdjVec& Zo, double alpha, double beta, double* partial,
+                   const AdjVec& Zb, const AdjVec& Nb, const AdjVec& Zo, double alpha, double beta, double fcoef,
+                   int accumulate, double* partial,
                    int nblk, hipStream_t st) {
-  k_adj_zb<T><<<dim3(nblk, 3, 1), 256, 0, st>>>(g, L, Y, Bpsi, bp0, bplane, dts, Zb, Nb, Zo, alpha, beta, partial);
+  k_adj_zb<T><<<dim3(nblk, 3, 1), 256, 0, st>>>(g, L, Y, Bpsi, bp0, bplane, dts, Zb, Nb, Zo, alpha, beta, fcoef,
+                                                accumulate, partial);
 }
 
-void launch_adj_sum(const double* partial, int nblk, double coef, double* out, hipStream_t st) {
-  k_adj_sum<<<1, 256, 0, st>>>(partial, nblk, coef, out);
+void launch_adj_sum(const double* partial, int nblk, int accumulate, double* out, hipStream_t st) {
+  k_adj_sum<<<1, kAdjSumThreads, 0, st>>>(partial, nblk, accumulate, out);
 }
 
 template <typename T>
@@ -494,8 +515,8 @@ void launch_adj_xb(const DGrid& g, const Layout& L, const AdjVec& X, const AdjVe
   template void launch_adj_rhs<T>(const DGrid&, const Layout&, const T*, const AdjVec&, double, T*, long, long,       \
                                   hipStream_t);                                                                      \
   template void launch_adj_zb<T>(const DGrid&, const Layout&, const AdjVec&, const T*, long, long, double,            \
-                                 const AdjVec&, const AdjVec&, const AdjVec&, double, double, double*, int,           \
-                                 hipStream_t);                                                                       \
+                                 const AdjVec&, const AdjVec&, const AdjVec&, double, double, double, int, double*,   \
+                                 int, hipStream_t);                                                                  \
   template void launch_adj_xb<T>(const DGrid&, const Layout&, const AdjVec&, const AdjVec&, const AdjVec&,            \
                                  const AdjVec&, T*, double, hipStream_t);
 

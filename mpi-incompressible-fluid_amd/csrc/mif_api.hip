@@ -570,10 +570,6 @@ int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[
 //   Nb, Sb/R1b from Y; Xb = Y + J_N(X)^T Nb      (stage update transposed)
 //   pbar += b_s G^T Y;  fbar += sum Nb
 // ---------------------------------------------------------------------------
-// workgroups per component of k_adj_zb: one point per thread up to this
-// many workgroups, grid-stride beyond
-constexpr int kAdjMaxBlocks = 4096;
-
 int adj_workspace(mif_ctx* c) {
   if (c->adj_partial) return MIF_OK;
   const size_t bytes = c->L.elems * c->esize;
@@ -585,7 +581,10 @@ int adj_workspace(mif_ctx* c) {
       HIPCHK(c, hipMemsetAsync(c->adj[q][cc], 0, bytes, c->stream));
     }
   HIPCHK(c, hipMalloc(&c->adj_partial, 3 * kAdjMaxBlocks * sizeof(double)));
-  HIPCHK(c, hipMalloc(&c->adj_fbar, 3 * sizeof(double)));
+  // [0..2]: the force adjoint of the last step; [3..5]: the accumulator of
+  // the steps called without a host force_bar (mif_adjoint_force_take)
+  HIPCHK(c, hipMalloc(&c->adj_fbar, 6 * sizeof(double)));
+  HIPCHK(c, hipMemsetAsync(c->adj_fbar, 0, 6 * sizeof(double), c->stream));
   return MIF_OK;
 }
 
@@ -601,7 +600,10 @@ AdjVec vec_ws(mif_ctx* c, int q) { return AdjVec{{c->adj[q][0], c->adj[q][1], c-
 //   x_bar = z_bar + J_N(X_s)^T N_bar, p_bar += b_s G^T z_bar   (k_adj_xb)
 // where y_bar is u_bar (stage 3) or the previous x_bar.  The force adjoint
 // is sum(N_bar) over the stages = (a3 - a2) S3 + (a2 + c21) S2 + a1 S1 with
-// S_s = sum(z_bar_s).
+// S_s = sum(z_bar_s): k_adj_zb accumulates the weighted block sums of the
+// three stages and one reduction per step adds them up.  force_bar == NULL:
+// the result is added to the context's device accumulator, with no
+// synchronisation.
 template <typename T>
 int step_adjoint(mif_ctx* c, mif_field* const un[3], mif_field* const y2[3], mif_field* const y3[3],
                  mif_field* const ubar[3], mif_field* pbar, double fbar[3]) {
@@ -631,17 +633,21 @@ int step_adjoint(mif_ctx* c, mif_field* const un[3], mif_field* const y2[3], mif
   T* B = (T*)c->B;
   const long interior_pts = (long)(g.N[0] + 1) * (g.N[1] + 1) * (g.N[2] + 1);
   const int nblk = (int)std::min<long>(kAdjMaxBlocks, (interior_pts + 255) / 256);
-  HIPCHK(c, hipMemsetAsync(c->adj_fbar, 0, 3 * sizeof(double), st));
   KTimer tm(c, MIF_KC_ADJOINT);
   for (int s = 3; s >= 1; s--) {
     const AdjVec Yin = s == 3 ? UB : XT;
     launch_adj_rhs<T>(g, L, P, Yin, dts[s], B, c->bp0, c->bplane, st);
     if ((rc = spectral_solve<T>(c, B))) return rc;
-    launch_adj_zb<T>(g, L, Yin, B, c->bp0, c->bplane, dts[s], zcur[s], NB, zlater[s], alpha[s], beta[s],
-                     c->adj_partial, nblk, st);
-    launch_adj_sum(c->adj_partial, nblk, fcoef[s], c->adj_fbar, st);
+    launch_adj_zb<T>(g, L, Yin, B, c->bp0, c->bplane, dts[s], zcur[s], NB, zlater[s], alpha[s], beta[s], fcoef[s],
+                     s == 3 ? 0 : 1, c->adj_partial, nblk, st);
     launch_adj_xb<T>(g, L, src[s], NB, zcur[s], s == 1 ? UB : XT, P, bcoef[s], st);
   }
+  if (!fbar) {
+    launch_adj_sum(c->adj_partial, nblk, 1, c->adj_fbar + 3, st);
+    LAUNCHCHK(c);
+    return MIF_OK;
+  }
+  launch_adj_sum(c->adj_partial, nblk, 0, c->adj_fbar, st);
   LAUNCHCHK(c);
   double fb[3];
   HIPCHK(c, hipMemcpyAsync(fb, c->adj_fbar, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1305,10 +1311,22 @@ int mif_timestep_adjoint(mif_field* const un[3], mif_field* const y2[3], mif_fie
   if ((rc = check_vel(un, &c)) || (rc = check_vel(y2, &c2)) || (rc = check_vel(y3, &c3)) ||
       (rc = check_vel(ubar, &c4)))
     return rc;
-  if (c2 != c || c3 != c || c4 != c || !pbar || pbar->ctx != c || pbar->stag != MIF_STAGGER_NONE || !force_bar)
+  if (c2 != c || c3 != c || c4 != c || !pbar || pbar->ctx != c || pbar->stag != MIF_STAGGER_NONE)
     return fail(c, MIF_ERR_INVALID_ARGUMENT, "inconsistent adjoint arguments");
   if (is_fp64(c)) return step_adjoint<double>(c, un, y2, y3, ubar, pbar, force_bar);
   return step_adjoint<float>(c, un, y2, y3, ubar, pbar, force_bar);
+}
+
+int mif_adjoint_force_take(mif_ctx* c, double force_bar[3]) {
+  if (!c || !force_bar) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  if (!c->adj_fbar) return MIF_OK;  // no adjoint step has run
+  HIPCHK(c, hipSetDevice(c->device));
+  double fb[3];
+  HIPCHK(c, hipMemcpyAsync(fb, c->adj_fbar + 3, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->adj_fbar + 3, 0, 3 * sizeof(double), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int q = 0; q < 3; q++) force_bar[q] += fb[q];
+  return MIF_OK;
 }
 
 int mif_field_copy(mif_field* dst, const mif_field* src) {

@@ -138,10 +138,14 @@ void launch_adj_rhs(const DGrid& g, const Layout& L, const T* P, const AdjVec& Y
 // of z_bar per component (partial holds 3 * nblk doubles).
 template <typename T>
 void launch_adj_zb(const DGrid& g, const Layout& L, const AdjVec& Y, const T* Bpsi, long bp0, long bplane, double dts,
-                   const AdjVec& Zb, const AdjVec& Nb, const AdjVec& Zo, double alpha, double beta, double* partial,
-                   int nblk, hipStream_t st);
-// out[c] += coef * sum of the partials of component c.
-void launch_adj_sum(const double* partial, int nblk, double coef, double* out, hipStream_t st);
+                   const AdjVec& Zb, const AdjVec& Nb, const AdjVec& Zo, double alpha, double beta, double fcoef,
+                   int accumulate, double* partial, int nblk, hipStream_t st);
+// Workgroups per component of k_adj_zb (one point per thread up to this
+// many, grid-stride beyond), and the number of partials k_adj_sum reduces.
+constexpr int kAdjMaxBlocks = 4096;
+// out[c] = (accumulate: out[c] +) the sum of the nblk <= kAdjMaxBlocks
+// partials of component c.
+void launch_adj_sum(const double* partial, int nblk, int accumulate, double* out, hipStream_t st);
 // x_bar = z_bar + J_N(X)^T N_bar; p_bar += bcoef * G^T z_bar.
 template <typename T>
 void launch_adj_xb(const DGrid& g, const Layout& L, const AdjVec& X, const AdjVec& Nb, const AdjVec& Zb,

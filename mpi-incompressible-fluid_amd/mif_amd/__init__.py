@@ -8,7 +8,8 @@ from ._lib import (FP64, MIXED, SOLVE_AUTO, SOLVE_FP32, SOLVE_FP64, STAGGER_NONE
                    FN_ES_PRESSURE, FN_ES_VELOCITY, FN_LID_1, FN_LID_2, FN_PT_GRAD_P, FN_PT_PRESSURE,
                    FN_PT_VELOCITY, FN_TIME_DIFFERENCE, FN_VT_FORCING, FN_VT_VELOCITY, FN_ZERO, KERNEL_CLASSES,
                    LIB_PATH, load)
-from .core import (Context, Field, GridSpec, MifError, VelocityField, adjust_pressure, error_norms_pressure,
+from .core import (Context, Field, GridSpec, MifError, VelocityField, adjoint_force_take, adjust_pressure,
+                   error_norms_pressure,
                    error_norms_velocity, function, poisson_solve, pressure_gradient,
                    solve_pressure_equation_homogeneous_periodic, solve_pressure_equation_non_homogeneous_neumann,
                    timestep, timestep_adjoint, timestep_nhn, timestep_velocity)

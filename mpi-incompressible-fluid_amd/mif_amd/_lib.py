@@ -112,6 +112,7 @@ SIGNATURES = {
     "mif_pressure_gradient": [_VP, _V3],
     "mif_ctx_set_body_force": [_VP, _DP],
     "mif_timestep_adjoint": [_V3, _V3, _V3, _V3, _VP, _DP],
+    "mif_adjoint_force_take": [_VP, _DP],
     "mif_field_copy": [_VP, _VP],
     "mif_field_dot": [_VP, _VP, _DP],
     "mif_ctx_set_timing": [_VP, _I],

@@ -359,16 +359,30 @@ def error_norms_pressure(pressure: Field, exact_pressure: L.MifFunction, t: floa
 
 
 def timestep_adjoint(u_n: VelocityField, y2: VelocityField, y3: VelocityField, u_bar: VelocityField,
-                     p_bar: Field, force_bar: np.ndarray):
+                     p_bar: Field, force_bar):
     """Adjoint of one timestep(): (u_bar, p_bar) go from the adjoints of
     (u^{n+1}, p^{n+1}) to those of (u^n, p^n); force_bar (3) accumulates.
     u_n, y2, y3 are the stencil sources of the forward step (u^n and the
-    buffers `velocity_buffer`, `velocity_buffer_2` as timestep() leaves them)."""
+    buffers `velocity_buffer`, `velocity_buffer_2` as timestep() leaves them).
+    force_bar=None: the force adjoint accumulates on the device (no
+    synchronisation) until adjoint_force_take()."""
     ctx = u_n.ctx
+    if force_bar is None:
+        ctx.check(ctx.lib.mif_timestep_adjoint(u_n.c_array(), y2.c_array(), y3.c_array(), u_bar.c_array(), p_bar.ptr,
+                                               None))
+        return
     fb = (ctypes.c_double * 3)(*[float(x) for x in force_bar])
     ctx.check(ctx.lib.mif_timestep_adjoint(u_n.c_array(), y2.c_array(), y3.c_array(), u_bar.c_array(), p_bar.ptr, fb))
     for a in range(3):
         force_bar[a] = fb[a]
+
+
+def adjoint_force_take(ctx: "Context") -> np.ndarray:
+    """The force adjoint accumulated by timestep_adjoint(..., None) calls
+    since the last take; resets the accumulator."""
+    fb = (ctypes.c_double * 3)(0.0, 0.0, 0.0)
+    ctx.check(ctx.lib.mif_adjoint_force_take(ctx.ptr, fb))
+    return np.array(list(fb))
 
 
 def pressure_gradient(pressure: Field, out: VelocityField):

@@ -50,7 +50,7 @@ def _oracle_adjoint(o, u0, p0, f, wu, wp, nsteps):
     return vel, p, ubar, pbar, fbar
 
 
-def _gpu_adjoint(gs, u0, p0, f, wu, wp, nsteps):
+def _gpu_adjoint(gs, u0, p0, f, wu, wp, nsteps, device_force=False):
     ctx = mif.Context(gs)
     ctx.set_body_force(f)
     vel, buf, buf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
@@ -71,8 +71,13 @@ def _gpu_adjoint(gs, u0, p0, f, wu, wp, nsteps):
     ubar.upload(wu)
     pbar.upload(wp)
     fbar = np.zeros(3)
+    if device_force:
+        mif.adjoint_force_take(ctx)
     for un, y2, y3 in reversed(rec):
-        mif.timestep_adjoint(un, y2, y3, ubar, pbar, fbar)
+        mif.timestep_adjoint(un, y2, y3, ubar, pbar, None if device_force else fbar)
+    if device_force:
+        fbar = mif.adjoint_force_take(ctx)
+        assert np.all(mif.adjoint_force_take(ctx) == 0.0), "the take resets the accumulator"
     return vel.download(), p.download(), ubar.download(), pbar.download(), fbar
 
 
@@ -100,6 +105,12 @@ def test_adjoint_matches_oracle(case):
         assert rel_err(gub[c], rub[c]) < 1e-11, ("u_bar", c, rel_err(gub[c], rub[c]))
     assert rel_err(gpb, rpb) < 1e-11, ("p_bar", rel_err(gpb, rpb))
     assert np.max(np.abs(gfb - rfb)) <= 1e-11 * np.max(np.abs(rfb)), (gfb, rfb)
+    # the device accumulator (force_bar = NULL, then a take) sums the same
+    # per-step values in the same order as the host
+    dv, dp, dub, dpb, dfb = _gpu_adjoint(gs, u0, p0, f, wu, wp, nsteps, device_force=True)
+    assert np.array_equal(dfb, gfb), (dfb, gfb)
+    for c in range(3):
+        assert np.array_equal(dub[c], gub[c]), ("u_bar, device force accumulation", c)
 
 
 def test_rollout_gradient_matches_central_differences():
