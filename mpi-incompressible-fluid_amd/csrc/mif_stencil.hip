@@ -90,10 +90,21 @@ constexpr int STN = STX * STY;                  // threads per workgroup
 constexpr int PLANE_ELEMS = 3 * SHY * SHX;      // one halo plane of u, v, w
 constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;  // plane elements per thread
 
+#ifndef MIF_STAGE_NS32
+#define MIF_STAGE_NS32 4
+#endif
+#ifndef MIF_STAGE_NS64
+#define MIF_STAGE_NS64 3
+#endif
+
 template <typename T, int KIND>
 __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax,
                                                int kmax, int ntx, int nty, int ntiles) {
-  __shared__ T pl[3][3][SHY][SHX];
+  // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 (plane
+  // k+2 is committed into the fourth slot while plane k is computed; one
+  // barrier per plane)
+  constexpr int NS = sizeof(T) == 4 ? MIF_STAGE_NS32 : MIF_STAGE_NS64;
+  __shared__ T pl[NS][3][SHY][SHX];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * STX + tx;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
@@ -137,9 +148,9 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
     fetch(k0 - 1, a);
     fetch(k0, b);
     fetch(k0 + 1, c);
-    commit((k0 - 1) % 3, a);
-    commit(k0 % 3, b);
-    commit((k0 + 1) % 3, c);
+    commit((k0 - 1) % NS, a);
+    commit(k0 % NS, b);
+    commit((k0 + 1) % NS, c);
   }
   T pre[PQ];
   if (k0 + 2 <= k1) fetch(k0 + 2, pre);
@@ -149,6 +160,11 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
 
   const long doff[3] = {1, L.PX, L.plane};
   for (int k = k0; k < k1; k++) {
+    if constexpr (NS == 4) {
+      // the slot of plane k-2, which no thread reads any more
+      if (k + 2 <= k1) commit((k + 2) % NS, pre);
+      if (k + 3 <= k1) fetch(k + 3, pre);
+    }
     if (in_tile) {
       const long o = L.idx(i, j, k);
       bool inside[3];
@@ -174,7 +190,7 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
             if constexpr (KIND == 5) bv[c] = ((const T*)vs.V[c])[o];
           }
       }
-      const int slot[3] = {(k + 2) % 3, k % 3, (k + 1) % 3};
+      const int slot[3] = {(k + NS - 1) % NS, k % NS, (k + 1) % NS};
       auto tap = [&](int f, int di, int dj, int dk) -> T { return pl[slot[dk + 1]][f][ty + 1 + dj][tx + 1 + di]; };
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -212,10 +228,12 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
       }
     }
     __syncthreads();
-    // plane k+2 replaces plane k-1; plane k+3 starts loading
-    if (k + 2 <= k1) commit((k + 2) % 3, pre);
-    if (k + 3 <= k1) fetch(k + 3, pre);
-    __syncthreads();
+    if constexpr (NS == 3) {
+      // plane k+2 replaces plane k-1; plane k+3 starts loading
+      if (k + 2 <= k1) commit((k + 2) % NS, pre);
+      if (k + 3 <= k1) fetch(k + 3, pre);
+      __syncthreads();
+    }
   }
 }
 
