@@ -209,6 +209,12 @@ int mif_adjoint_force_take(mif_ctx *ctx, double force_bar[3]);
 int mif_field_copy(mif_field *dst, const mif_field *src);
 /* Sum over all points of the extents of a * b (FP64 accumulation). */
 int mif_field_dot(const mif_field *a, const mif_field *b, double *out);
+/* Values of one field at n points of its extents, given as (i, j, k)
+ * triples in ijk[3n], written as FP64 into out[n].  No counterpart in the
+ * reference, which reads its host tensors directly: the output writers
+ * (writeVTK/writeDat, src/VTKDatExport.cpp) fetch only the points they
+ * print through this.  Allocates device scratch per call; synchronises. */
+int mif_field_gather(const mif_field *f, const int32_t *ijk, int64_t n, double *out);
 
 /* ---- measurement ------------------------------------------------------------
  * Kernel timing with HIP events recorded on the context's stream around

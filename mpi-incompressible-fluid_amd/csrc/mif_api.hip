@@ -1357,6 +1357,38 @@ int mif_field_dot(const mif_field* a, const mif_field* b, double* out) {
   return MIF_OK;
 }
 
+int mif_field_gather(const mif_field* f, const int32_t* ijk, int64_t n, double* out) {
+  if (!f || !f->ctx || n < 0 || (n > 0 && (!ijk || !out)))
+    return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad gather arguments");
+  mif_ctx* c = f->ctx;
+  if (n == 0) return MIF_OK;
+  int e[3];
+  extents(c->g, f->stag, e);
+  for (int64_t q = 0; q < n; q++)
+    for (int a = 0; a < 3; a++)
+      if (ijk[3 * q + a] < 0 || ijk[3 * q + a] >= e[a])
+        return fail(c, MIF_ERR_INVALID_ARGUMENT, "gather point %lld outside the field", (long long)q);
+  HIPCHK(c, hipSetDevice(c->device));
+  int* dijk = nullptr;
+  double* dout = nullptr;
+  hipError_t err = hipMalloc(&dijk, 3 * n * sizeof(int));
+  if (err == hipSuccess) err = hipMalloc(&dout, n * sizeof(double));
+  if (err == hipSuccess) err = hipMemcpyAsync(dijk, ijk, 3 * n * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  if (err == hipSuccess) {
+    if (is_fp64(c))
+      launch_gather<double>(c->L, (const double*)f->data, dijk, (long)n, dout, c->stream);
+    else
+      launch_gather<float>(c->L, (const float*)f->data, dijk, (long)n, dout, c->stream);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+  (void)hipFree(dijk);
+  (void)hipFree(dout);
+  if (err != hipSuccess) return fail(c, MIF_ERR_HIP, "gather: %s", hipGetErrorString(err));
+  return MIF_OK;
+}
+
 int mif_ctx_set_timing(mif_ctx* c, int enable) {
   if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
   drain_timers(c);

@@ -94,6 +94,9 @@ void launch_copy(long n, const T* src, T* dst, hipStream_t st);
 template <typename T>
 void launch_dot(const Layout& L, const int e[3], const T* a, const T* b, Red3* partial, int nblocks, Red3* out,
                 hipStream_t st);
+// out[q] = F(ijk[3q], ijk[3q+1], ijk[3q+2]) for q < n (indices checked by the caller).
+template <typename T>
+void launch_gather(const Layout& L, const T* F, const int* ijk, long n, double* out, hipStream_t st);
 template <typename T>
 void launch_add_const(const DGrid& g, const Layout& L, T* P, double c, hipStream_t st);
 // Exact-solution tables for the norms and adjust_pressure: node-shaped

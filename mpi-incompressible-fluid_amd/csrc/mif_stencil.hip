@@ -871,6 +871,20 @@ void launch_dot(const Layout& L, const int e[3], const T* a, const T* b, Red3* p
   k_reduce_final<<<1, 256, 0, st>>>(partial, nblocks, out);
 }
 
+// Values of one field at a list of points (i, j, k), as FP64.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather(Layout L, const T* __restrict__ F, const int* __restrict__ ijk,
+                                                long n, double* __restrict__ out) {
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  if (q >= n) return;
+  out[q] = (double)F[L.idx(ijk[3 * q], ijk[3 * q + 1], ijk[3 * q + 2])];
+}
+
+template <typename T>
+void launch_gather(const Layout& L, const T* F, const int* ijk, long n, double* out, hipStream_t st) {
+  k_gather<T><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(L, F, ijk, n, out);
+}
+
 #define MIF_INSTANTIATE(T)                                                                                          \
   template void launch_stage<T>(int, const DGrid&, const Layout&, const StageArgs&, hipStream_t);                   \
   template void launch_vel_stage<T>(int, const DGrid&, const Layout&, const VelStageArgs&, hipStream_t);            \
@@ -883,6 +897,7 @@ void launch_dot(const Layout& L, const int e[3], const T* a, const T* b, Red3* p
   template void launch_to_dense<T>(const Layout&, const int*, const T*, double*, hipStream_t);                      \
   template void launch_copy<T>(long, const T*, T*, hipStream_t);                                                    \
   template void launch_dot<T>(const Layout&, const int*, const T*, const T*, Red3*, int, Red3*, hipStream_t);       \
+  template void launch_gather<T>(const Layout&, const T*, const int*, long, double*, hipStream_t);                 \
   template void launch_add_const<T>(const DGrid&, const Layout&, T*, double, hipStream_t);                          \
   template void launch_norm_velocity<T>(const DGrid&, const Layout&, const T*, const T*, const T*, const FnDesc&,   \
                                         double, const ExactTables&, Red3*, int, Red3*, hipStream_t);                \

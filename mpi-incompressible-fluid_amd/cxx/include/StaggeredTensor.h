@@ -14,7 +14,9 @@
 
 #include <array>
 #include <cstddef>
+#include <cstdint>
 #include <functional>
+#include <string>
 #include <vector>
 
 #include "Constants.h"
@@ -85,6 +87,23 @@ class StaggeredTensor {
   // Exchange the values with another tensor of the same staggering in
   // constant time (device pointers and host mirrors).
   void swap_data(StaggeredTensor &other);
+
+  // Text dump and load (Tensor::dump / Tensor::load, include/Tensor.h:433-476):
+  // dump writes "Tensor (row major). Dimensions: [ sx sy sz ]" and the values
+  // in storage order (x fastest) between brackets, with the stream's default
+  // formatting (6 significant digits, as the reference prints them); load
+  // reads whitespace-separated values in storage order, either a plain list
+  // or a dump's body (the header is skipped), and leaves the remaining values
+  // unchanged when the file holds fewer.  Both return 1, or print "Error
+  // opening <file> file." and return 0.
+  uint8_t dump(const std::string &file_name) const;
+  uint8_t load(const std::string &file_name);
+  // Exact restart files: a 48-byte header ("MIFTNSR1", the three extents and
+  // the staggering as 64-bit integers, the value size) followed by the
+  // values as little-endian doubles in storage order.  load_binary checks
+  // the extents and the staggering; both return 1 on success, 0 otherwise.
+  uint8_t dump_binary(const std::string &file_name) const;
+  uint8_t load_binary(const std::string &file_name);
 
   // The device field, with any newer host values uploaded first.
   mif_field *device() const;

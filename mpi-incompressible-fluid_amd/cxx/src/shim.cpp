@@ -2,7 +2,11 @@
 // the HIP library.  Every numerical routine runs on the device; the host
 // side keeps the reference's objects, converts callbacks to descriptors and
 // tabulates callbacks that have none.
+#include <cstring>
+#include <fstream>
+#include <iostream>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 
@@ -171,6 +175,96 @@ void StaggeredTensor::swap_data(StaggeredTensor &other) {
   check_status(mif_field_swap(field_, other.field_), constants.context(), "mif_field_swap");
   host_.swap(other.host_);
   std::swap(state_, other.state_);
+}
+
+// ---- dump / load -----------------------------------------------------------------
+uint8_t StaggeredTensor::dump(const std::string &file_name) const {
+  std::ofstream out(file_name);
+  if (!out) {
+    std::cerr << "Error opening " << file_name << " file." << std::endl;
+    return 0;
+  }
+  pull();
+  out << "Tensor (row major). Dimensions: [ ";
+  for (size_t v : sizes_) out << v << " ";
+  out << "]" << std::endl << "[ ";
+  for (const Real v : host_) out << v << " ";
+  out << "]" << std::endl;
+  return 1;
+}
+
+uint8_t StaggeredTensor::load(const std::string &file_name) {
+  std::ifstream in(file_name);
+  if (!in) {
+    std::cerr << "Error opening " << file_name << " file." << std::endl;
+    return 0;
+  }
+  std::stringstream whole;
+  whole << in.rdbuf();
+  std::string text = whole.str();
+  // a dump's header ends with the dimensions' closing bracket; its values
+  // follow the next opening bracket
+  if (text.compare(0, 6, "Tensor") == 0) {
+    const size_t close = text.find(']');
+    const size_t open = close == std::string::npos ? close : text.find('[', close);
+    text = open == std::string::npos ? std::string() : text.substr(open + 1);
+  }
+  pull();
+  std::istringstream values(text);
+  for (size_t q = 0; q < host_.size(); q++) {
+    Real v;
+    if (!(values >> v)) break;
+    host_[q] = v;
+  }
+  state_ = HOST_NEWER;
+  return 1;
+}
+
+namespace {
+constexpr char kBinaryMagic[8] = {'M', 'I', 'F', 'T', 'N', 'S', 'R', '1'};
+}
+
+uint8_t StaggeredTensor::dump_binary(const std::string &file_name) const {
+  std::ofstream out(file_name, std::ios::binary);
+  if (!out) {
+    std::cerr << "Error opening " << file_name << " file." << std::endl;
+    return 0;
+  }
+  pull();
+  const int64_t head[5] = {static_cast<int64_t>(sizes_[0]), static_cast<int64_t>(sizes_[1]),
+                           static_cast<int64_t>(sizes_[2]), static_cast<int64_t>(staggering),
+                           static_cast<int64_t>(sizeof(Real))};
+  out.write(kBinaryMagic, sizeof(kBinaryMagic));
+  out.write(reinterpret_cast<const char *>(head), sizeof(head));
+  out.write(reinterpret_cast<const char *>(host_.data()), static_cast<std::streamsize>(host_.size() * sizeof(Real)));
+  return out ? 1 : 0;
+}
+
+uint8_t StaggeredTensor::load_binary(const std::string &file_name) {
+  std::ifstream in(file_name, std::ios::binary);
+  if (!in) {
+    std::cerr << "Error opening " << file_name << " file." << std::endl;
+    return 0;
+  }
+  char magic[8];
+  int64_t head[5];
+  in.read(magic, sizeof(magic));
+  in.read(reinterpret_cast<char *>(head), sizeof(head));
+  if (!in || std::memcmp(magic, kBinaryMagic, sizeof(magic)) != 0 || head[0] != static_cast<int64_t>(sizes_[0]) ||
+      head[1] != static_cast<int64_t>(sizes_[1]) || head[2] != static_cast<int64_t>(sizes_[2]) ||
+      head[3] != static_cast<int64_t>(staggering) || head[4] != static_cast<int64_t>(sizeof(Real))) {
+    std::cerr << file_name << ": not a restart file of a tensor with these extents and staggering" << std::endl;
+    return 0;
+  }
+  std::vector<Real> values(host_.size());
+  in.read(reinterpret_cast<char *>(values.data()), static_cast<std::streamsize>(values.size() * sizeof(Real)));
+  if (!in) {
+    std::cerr << file_name << ": truncated restart file" << std::endl;
+    return 0;
+  }
+  host_.swap(values);
+  state_ = HOST_NEWER;
+  return 1;
 }
 
 // ---- VelocityTensor ------------------------------------------------------------

@@ -4,7 +4,8 @@ Compiles every HIP source of csrc/ to an object in parallel and links
 _build/libmif_gpu.so in-tree, so the built library travels with the source
 tree.  Then compiles the C++ host interface of cxx/ (the reference's class
 and function signatures over the C ABI) into _build/libmif_cxx.a and links
-the C++ driver programs of tests/cpp/ against both.  Invoked by
+against both the `mif` program (the reference's executable) and the C++
+driver programs of tests/cpp/.  Invoked by
 __graft_entry__.build() and usable as `python -m mif_amd.build` from the
 package directory.
 """
@@ -24,10 +25,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ROOT = os.path.dirname(PKG_DIR)
 CXX_DIR = os.path.join(PKG_DIR, "cxx")
-CXX_SOURCES = ["shim.cpp", "functions.cpp"]
+CXX_SOURCES = ["shim.cpp", "functions.cpp", "InputParser.cpp", "VTKDatExport.cpp"]
 CXX_LIB = os.path.join(OUT, "libmif_cxx.a")
 DRIVER_DIR = os.path.join(ROOT, "tests", "cpp")
-DRIVERS = ["full_test", "velocity_test", "pressure_test"]
+DRIVERS = ["full_test", "velocity_test", "pressure_test", "io_test"]
+# programs of the package itself (the reference's `mif` executable)
+PROGRAMS = {"mif": "mif_main.cpp"}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra",
              "-I", os.path.join(CXX_DIR, "include"), "-I", os.path.join(ROOT, "include")]
 FLAGS = [
@@ -78,10 +81,12 @@ def build_cxx() -> str:
     if os.path.exists(CXX_LIB):
         os.remove(CXX_LIB)
     _run(["ar", "rcs", CXX_LIB, *objs], "ar")
-    for name in DRIVERS:
+    sources = {name: os.path.join(DRIVER_DIR, name + ".cpp") for name in DRIVERS}
+    sources.update({name: os.path.join(CXX_DIR, "src", src) for name, src in PROGRAMS.items()})
+    for name, src in sources.items():
         exe = os.path.join(OUT, name)
-        _run([CXX, *CXX_FLAGS, os.path.join(DRIVER_DIR, name + ".cpp"), "-o", exe + ".tmp", CXX_LIB,
-              "-L", OUT, "-lmif_gpu", "-Wl,-rpath,$ORIGIN"], f"{CXX} {name}")
+        _run([CXX, *CXX_FLAGS, src, "-o", exe + ".tmp", CXX_LIB, "-L", OUT, "-lmif_gpu", "-Wl,-rpath,$ORIGIN"],
+             f"{CXX} {name}")
         os.replace(exe + ".tmp", exe)
     return CXX_LIB
 

@@ -244,6 +244,16 @@ class Field:
         self.ctx.check(self.ctx.lib.mif_field_dot(self.ptr, other.ptr, ctypes.byref(out)))
         return out.value
 
+    def gather(self, ijk) -> np.ndarray:
+        """Values at points (i, j, k) of the extents (an (n, 3) integer array),
+        fetched from the device without downloading the field."""
+        idx = np.ascontiguousarray(np.asarray(ijk, dtype=np.int32).reshape(-1, 3))
+        out = np.empty(idx.shape[0])
+        self.ctx.check(self.ctx.lib.mif_field_gather(
+            self.ptr, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), idx.shape[0],
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     def set(self, fn: L.MifFunction, comp: int = 0, t: float = 0.0, include_border: bool = True):
         self.ctx.check(self.ctx.lib.mif_field_set(self.ptr, ctypes.byref(fn), comp, t, 1 if include_border else 0))
 
