@@ -228,6 +228,61 @@ class Field:
         self.ctx.check(self.ctx.lib.mif_field_download(self.ptr, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    # ---- restart files (Tensor::dump / Tensor::load, include/Tensor.h:433-476,
+    # and the exact binary variant of the C++ interface's StaggeredTensor) ----
+    _BIN_MAGIC = b"MIFTNSR1"
+
+    def dump(self, path: str):
+        """Text dump in the reference's format: the dimensions line, then the
+        values in storage order (x fastest) with 6 significant digits."""
+        sx = self.shape[::-1]
+        vals = self.download().ravel()
+        with open(path, "w") as f:
+            f.write("Tensor (row major). Dimensions: [ %d %d %d ]\n[ " % sx)
+            f.write("".join("%g " % v for v in vals))
+            f.write("]\n")
+
+    def load(self, path: str):
+        """Values from a text dump or a plain whitespace-separated list, in
+        storage order; points the file does not reach keep their values."""
+        text = open(path).read()
+        if text.startswith("Tensor"):
+            close = text.find("]")
+            start = text.find("[", close) if close >= 0 else -1
+            text = text[start + 1:] if start >= 0 else ""
+        vals = []
+        for tok in text.split():
+            try:
+                vals.append(float(tok))
+            except ValueError:
+                break
+        cur = self.download().ravel()
+        n = min(len(vals), cur.size)
+        cur[:n] = vals[:n]
+        self.upload(cur.reshape(self.shape))
+
+    def dump_binary(self, path: str):
+        """Exact restart file: 'MIFTNSR1', the extents (x, y, z), the
+        staggering and the value size as little-endian int64, then the values
+        as little-endian doubles in storage order."""
+        sx = self.shape[::-1]
+        head = np.array([sx[0], sx[1], sx[2], self.staggering, 8], dtype="<i8")
+        with open(path, "wb") as f:
+            f.write(self._BIN_MAGIC)
+            f.write(head.tobytes())
+            f.write(self.download().astype("<f8").tobytes())
+
+    def load_binary(self, path: str):
+        data = open(path, "rb").read()
+        sx = self.shape[::-1]
+        head = np.frombuffer(data, dtype="<i8", count=5, offset=8) if len(data) >= 48 else None
+        if data[:8] != self._BIN_MAGIC or head is None or list(head) != [sx[0], sx[1], sx[2], self.staggering, 8]:
+            raise ValueError(f"{path}: not a restart file of a field with these extents and staggering")
+        n = int(np.prod(self.shape))
+        if len(data) != 48 + 8 * n:
+            raise ValueError(f"{path}: truncated restart file")
+        self.upload(np.frombuffer(data, dtype="<f8", count=n, offset=48).reshape(self.shape))
+
     def upload_device(self, ptr: int):
         """Copy from a device array (reference layout, FP64) at address ptr."""
         self.ctx.check(self.ctx.lib.mif_field_upload_device(self.ptr, ctypes.c_void_p(ptr)))

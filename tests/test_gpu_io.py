@@ -97,6 +97,26 @@ def test_writers_and_restart_files(tmp_path, pz):
     text = (tmp_path / "u.txt").read_text().split("\n")
     assert text[0] == "Tensor (row major). Dimensions: [ %d %d %d ]" % G.ext["u"]
 
+    # the Python binding reads and writes the same files
+    import mif_amd as mif
+
+    gs = mif.GridSpec(Nx_global=N, Ny_global=N, Nz_global=N, x_size=1.0, y_size=1.0, z_size=1.0 if pz else 2.0,
+                      min_x=-0.5 if pz else 0.0, min_y=-0.5 if pz else 0.0, min_z=-0.5 if pz else -1.0, Re=1e3,
+                      final_time=1e-3, num_time_steps=1, periodic=(0, 0, pz))
+    ctx = mif.Context(gs)
+    pf, uf = ctx.pressure(), ctx.velocity().components[0]
+    pf.load_binary(str(tmp_path / "p.bin"))
+    assert np.array_equal(pf.download(), p)
+    with pytest.raises(ValueError):
+        uf.load_binary(str(tmp_path / "p.bin"))
+    uf.load(str(tmp_path / "u.txt"))
+    assert np.allclose(uf.download(), u, rtol=1e-5, atol=1e-5)
+    uf.dump(str(tmp_path / "u_py.txt"))
+    assert (tmp_path / "u_py.txt").read_text() == (tmp_path / "u.txt").read_text()
+    pf.dump_binary(str(tmp_path / "p_py.bin"))
+    assert (tmp_path / "p_py.bin").read_bytes() == (tmp_path / "p.bin").read_bytes()
+    ctx.close()
+
 
 def _oracle_lid(N, Nt, dt, case2):
     Re = 1e3
