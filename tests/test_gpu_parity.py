@@ -189,6 +189,25 @@ def test_full_timestep_channel_radix3_fused_path():
     assert rel_err(gp, p) < TOL
 
 
+@pytest.mark.parametrize("ny", [129, 513])
+def test_full_timestep_channel_long_wall_normal_axis(ny):
+    # Neumann y of 129 nodes (the 2N-point DCT-I kernel) and of 513 nodes
+    # (the DCT-I through a 512-point FFT of two lines).  The pressure
+    # accumulates the solves' rounding times the Laplacian's condition number
+    # (~1.6e4 at 129 nodes, 16x that of the 33-node channels above; ~2.8e5 at
+    # 513): any two FP64 solve orderings differ by ~4e-11 relative in p at
+    # 129 nodes (both transform algorithms measured), so pressure is held to
+    # 1e-11 times kappa / 1e3 and the velocity, which sees only gradients, to
+    # the 1e-11 bar.
+    _print_case(f"timestep channel 9x{ny}x9")
+    o, vel, p, gvel, gp = _run_full((9, ny, 9), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 5e-3, 3, (1, 0, 1),
+                                    "channel", mo.FN_ZERO, seed=9)
+    for c in range(3):
+        assert rel_err(gvel[c], vel[c]) < TOL, c
+    kappa = (4.0 / (2.0 / (ny - 1)) ** 2) / (2 * (1 - np.cos(2 * PI / 8)) / (2 * PI / 8) ** 2)
+    assert rel_err(gp, p) < TOL * kappa / 1e3, (rel_err(gp, p), kappa)
+
+
 def test_full_timestep_walls_pow2_fused_path():
     _print_case("timestep ES 33^3 walls")
     o, vel, p, gvel, gp = _run_full((33, 33, 33), (1.0, 1.0, 2.0), (0.0, 0.0, -1.0), 1e3, 1e-4, 2, (0, 0, 0), "es",
