@@ -397,9 +397,27 @@ int pressure_solve_mixed(mif_ctx* c, mif_field* p, mif_field* const vel[3], doub
 
 // Mixed form with FP64 transforms: FP64 divergence of the FP32 velocity,
 // the FP64 spectral solve on the FP64 packed buffer, unpack into the FP32
-// increment.
+// increment.  On x lengths with a specialised transform the divergence and
+// the unpack are fused into the FP64 x passes, which read and write the
+// FP32 fields.
 int pressure_solve_mixed64(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
                            int nhn) {
+  if (!nhn && fft_fast_len(c->plan[0].L)) {
+    FusedIO io{vel[0]->data, vel[1]->data, vel[2]->data, p->data, c->L, dt};
+    io.fields_fp32 = 1;
+    const SpectralScale sc = scale_of<double>(c);
+    {
+      KTimer t(c, MIF_KC_FFT_X_DIV);
+      launch_fft_pow2<double>(c->g, 0, 0, c->plan[0], sc, c->B64, c->bp0, c->bplane, &io, c->stream);
+    }
+    spectral_middle<double>(c, c->B64);
+    {
+      KTimer t(c, MIF_KC_FFT_X_UNPACK);
+      launch_fft_pow2<double>(c->g, 0, 1, c->plan[0], sc, c->B64, c->bp0, c->bplane, &io, c->stream);
+    }
+    LAUNCHCHK(c);
+    return MIF_OK;
+  }
   {
     KTimer t(c, MIF_KC_DIV);
     launch_div_rhs<float, double>(c->g, c->L, fptr<float>(vel[0]), fptr<float>(vel[1]), fptr<float>(vel[2]), c->B64,

@@ -163,19 +163,24 @@ __device__ inline void fft(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
 using p2::Cx;
 using p2::pd;
 
+#ifndef MIF_TWL_MAX
+#define MIF_TWL_MAX 65536
+#endif
+
 // MODE 0: forward, 1: inverse, 2: forward + scale + inverse (axis 2 only).
 // FUSED (axis 0 only): MODE 0 reads div(u)/dt from the velocity instead of
 // B (compute_rhs_homogeneous_periodic, src/PressureEquation.cpp:59-61);
 // MODE 1 writes the solution into the pressure field, owner points and
 // periodic ghost images (PressureTensor::copy_to_staggered,
 // src/PressureTensor.cpp:21-34), instead of B.
-template <typename T, int L, int NP, int AXIS, int MODE, bool FUSED>
+// TV: element type of the velocity and pressure fields of the fused passes.
+template <typename T, int L, int NP, int AXIS, int MODE, bool FUSED, typename TV = T>
 __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W, SpectralScale sc, T* __restrict__ B,
                                                 long bp0, long bplane, int periodic, FusedIO io) {
-  // twiddles live in LDS when the workgroup stays within 64 KB, otherwise
-  // they are read from the (L1-resident) global table
+  // twiddles live in LDS when the workgroup stays within MIF_TWL_MAX
+  // bytes, otherwise they are read from the (L1-resident) global table
   constexpr int LP = p2::lds_pitch(L);
-  constexpr bool TWL = (NP * LP + L) * sizeof(Cx<T>) <= 65536;
+  constexpr bool TWL = (NP * LP + L) * sizeof(Cx<T>) <= MIF_TWL_MAX;
   static_assert(NP * LP * sizeof(Cx<T>) + (L / 2 + 1) * sizeof(double) <= 65536 - (TWL ? L * sizeof(Cx<T>) : 0),
                 "workgroup LDS above 64 KB");
   __shared__ Cx<T> Z[NP][LP];
@@ -235,9 +240,9 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   // chunk's loads are in flight together before its values go to LDS.
   constexpr int R0 = (L + 255) / 256;  // elements per thread and line on axis 0
   if (AXIS == 0 && FUSED && MODE == 0) {
-    const T* u = (const T*)io.u;
-    const T* v = (const T*)io.v;
-    const T* w = (const T*)io.w;
+    const TV* u = (const TV*)io.u;
+    const TV* v = (const TV*)io.v;
+    const TV* w = (const TV*)io.w;
     const long PX = io.L.PX, PL = io.L.plane;
     constexpr int CL0 = R0 >= 4 ? 1 : 4 / R0;  // lines per chunk (6 loads per value)
     constexpr int CL = CL0 < NL ? CL0 : NL;
@@ -254,7 +259,7 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
           const int m = t + 256 * q;
           const bool okm = okl && m < n;
           const long o = okm ? io.L.idx(m + g.olo[0], j, k) : 0;
-          const T u0 = u[o], u1 = u[o + 1], v0 = v[o], v1 = v[o + PX], w0 = w[o], w1 = w[o + PL];
+          const TV u0 = u[o], u1 = u[o + 1], v0 = v[o], v1 = v[o + PX], w0 = w[o], w1 = w[o + PL];
           const double du_dx = ((double)u1 - (double)u0) * g.one_over_h[0];
           const double dv_dy = ((double)v1 - (double)v0) * g.one_over_h[1];
           const double dw_dz = ((double)w1 - (double)w0) * g.one_over_h[2];
@@ -404,7 +409,7 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   } else {
     // DCT-I coefficients, or the real lines of an inverse transform
     if (AXIS == 0 && FUSED && MODE == 1) {
-      T* P = (T*)io.P;
+      TV* P = (TV*)io.P;
       const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
       for (int l = 0; l < NL; l++) {
         const int r = bx * NL + l;
@@ -416,7 +421,7 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
         if (g.periodic[2]) ks[1] = k == 1 ? N2 - 1 : (k == N2 - 2 ? 0 : -1);
         for (int m = t; m < n; m += 256) {
           const Cx<T> a = Z[l % NP][pd(m)];
-          const T val = l < NP ? a.re : a.im;
+          const TV val = (TV)(l < NP ? a.re : a.im);
           const int i = m + g.olo[0];
           const int i2 = g.periodic[0] ? (i == 1 ? N0 - 1 : (i == N0 - 2 ? 0 : -1)) : -1;
           for (int qk = 0; qk < 2; qk++) {
@@ -771,7 +776,12 @@ static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, c
   const int per = plan.periodic;
 #define MIF_P2(AX, MD) k_fft_p2<T, L, NP, AX, MD, false><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, none)
   if constexpr (PASSES & 1) {
-    if (axis == 0 && io) {
+    if (axis == 0 && io && sizeof(T) == 8 && io->fields_fp32) {
+      if (mode == 0)
+        k_fft_p2<T, L, NP, 0, 0, true, float><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+      else
+        k_fft_p2<T, L, NP, 0, 1, true, float><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+    } else if (axis == 0 && io) {
       if (mode == 0)
         k_fft_p2<T, L, NP, 0, 0, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
       else

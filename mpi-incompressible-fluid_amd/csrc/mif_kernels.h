@@ -61,6 +61,9 @@ struct FusedIO {
   void* P;
   Layout L;
   double dt;
+  // the velocity and pressure fields hold FP32 values while the transform
+  // runs in FP64 (the mixed form with FP64 transforms)
+  int fields_fp32 = 0;
 };
 
 template <typename T>

@@ -117,12 +117,16 @@ def test_solve_precision_selection():
     ctx.close()
 
 
-def test_mixed_with_fp64_transforms_tracks_fp64():
+@pytest.mark.parametrize("nx", [32, 64])
+def test_mixed_with_fp64_transforms_tracks_fp64(nx):
     """FP32 fields and stencils with FP64 transforms stay within FP32
-    rounding of the FP64 form."""
+    rounding of the FP64 form: x length 32 (separate divergence and unpack
+    kernels) and 64 (both fused into the FP64 x passes, which read and write
+    the FP32 fields)."""
     from mif_amd.problems import channel_grid, channel_initial_velocity
 
-    gs = channel_grid(32, 32, 32, Re=1000.0, dt=5e-4, steps=5)
+    print(f"[case] mixed with FP64 transforms, {nx}x32x32", flush=True)
+    gs = channel_grid(nx, 32, 32, Re=1000.0, dt=5e-4, steps=5)
     init = channel_initial_velocity(gs)
     out = {}
     for name, prec in (("fp64", mif.FP64), ("mixed64", mif.MIXED)):
