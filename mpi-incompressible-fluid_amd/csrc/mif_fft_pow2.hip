@@ -88,6 +88,10 @@ __device__ inline void dft(Cx<T>* v) {
   }
 }
 
+#ifndef MIF_TW_POW
+#define MIF_TW_POW 1
+#endif
+
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 // L = 2^m or 3 * 2^m: stages of radix 8, 8, ..., then 4 or 2, then 3
 constexpr bool has3(int L) { return L % 3 == 0; }
@@ -126,12 +130,31 @@ __device__ inline void stage(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
 #pragma unroll
       for (int r = 0; r < R; r++) v[b][r] = Z[p][pd(jb + r * m)];
       if (Ns > 1) {
+#if MIF_TW_POW
+        // one twiddle from the table, the others as its powers (products at
+        // most three deep: w2 = w1 w1, w4 = w2 w2, w3 = w1 w2, w5 = w1 w4,
+        // w6 = w2 w4, w7 = w3 w4), one LDS read per butterfly instead of R-1
+        Cx<T> w[R];
+        w[1] = tw[k * tstep];
+        if (SIGN > 0) w[1].im = -w[1].im;
+        if constexpr (R >= 3) w[2] = mul(w[1], w[1]);
+        if constexpr (R >= 4) w[3] = mul(w[1], w[2]);
+        if constexpr (R == 8) {
+          w[4] = mul(w[2], w[2]);
+          w[5] = mul(w[1], w[4]);
+          w[6] = mul(w[2], w[4]);
+          w[7] = mul(w[3], w[4]);
+        }
+#pragma unroll
+        for (int r = 1; r < R; r++) v[b][r] = mul(v[b][r], w[r]);
+#else
 #pragma unroll
         for (int r = 1; r < R; r++) {
           Cx<T> w = tw[r * k * tstep];
           if (SIGN > 0) w.im = -w.im;
           v[b][r] = mul(v[b][r], w);
         }
+#endif
       }
     }
   }
