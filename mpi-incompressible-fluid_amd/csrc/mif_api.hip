@@ -63,6 +63,10 @@ struct mif_ctx {
   void* adj[4][3]{};
   double* adj_partial = nullptr;
   double* adj_fbar = nullptr;
+  // velocity-only step: sin/cos of the coordinates of every axis,
+  // unshifted and staggered (launch_trig_tables), filled on first use
+  double* trig = nullptr;
+  int trig_n = 0;
   // transfers and reductions
   double* staging = nullptr;
   long staging_elems = 0;
@@ -546,6 +550,16 @@ int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[
   VelStageArgs s{};
   s.dt = dt;
   s.forcing = fo;
+  if (fo.kind == MIF_FN_VT_FORCING) {
+    if (!c->trig) {
+      const int tn = std::max(c->g.N[0], std::max(c->g.N[1], c->g.N[2])) + 1;
+      HIPCHK(c, hipMalloc(&c->trig, 12 * (size_t)tn * sizeof(double)));
+      c->trig_n = tn;
+      launch_trig_tables(c->g, c->trig, tn, c->stream);
+    }
+    s.trig = c->trig;
+    s.tn = c->trig_n;
+  }
   for (int q = 0; q < 3; q++) {
     s.V[q] = vel[q]->data;
     s.R[q] = rhs[q]->data;
@@ -846,6 +860,7 @@ int mif_ctx_destroy(mif_ctx* c) {
     for (int cc = 0; cc < 3; cc++) (void)hipFree(c->adj[q][cc]);
   (void)hipFree(c->adj_partial);
   (void)hipFree(c->adj_fbar);
+  (void)hipFree(c->trig);
   for (int a = 0; a < 3; a++) {
     (void)hipFree(c->W64[a]);
     (void)hipFree(c->W32[a]);

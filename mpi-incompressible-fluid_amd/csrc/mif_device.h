@@ -102,6 +102,16 @@ __device__ inline double es_dpdx(double t, double x, double y, double z, double 
   return -a * a / 2 * s * exp(-2 * d * d * t / Re);
 }
 
+// Forcing of the velocity-only manufactured field: f = u_t + (u.grad)u -
+// lap(u)/Re of MIF_FN_VT_VELOCITY (generators/manufsol_velocity.py:40-50),
+// simplified, from the sines and cosines of the coordinates and the time.
+__host__ __device__ inline double vt_forcing(int c, double sx, double cx, double sy, double cy, double sz, double cz,
+                                             double st, double ct, double Re) {
+  if (c == 0) return sx * cx * st * st * (2 - 2 * sy * sy - sz * sz) + sx * sz * cy * (ct + 3 * st / Re);
+  if (c == 1) return sy * cy * st * st * (2 - 2 * sx * sx - sz * sz) + sy * sz * cx * (ct + 3 * st / Re);
+  return sz * cz * st * st * (2 * sx * sx + 2 * sy * sy - 4) + cx * cy * cz * (2 * ct + 6 * st / Re);
+}
+
 // Kept out of line: the closed forms are large and only the boundary,
 // forcing and set-up kernels evaluate them.
 __device__ __noinline__ double fn_base(const FnDesc& f, int kind, int c, double t, double x, double y, double z) {
@@ -119,15 +129,8 @@ __device__ __noinline__ double fn_base(const FnDesc& f, int kind, int c, double 
       if (c == 0) return sin(x) * cos(y) * sin(z) * sin(t);
       if (c == 1) return cos(x) * sin(y) * sin(z) * sin(t);
       return 2 * cos(x) * cos(y) * cos(z) * sin(t);
-    case MIF_FN_VT_FORCING: {
-      // f = u_t + (u.grad)u - lap(u)/Re of the field above
-      // (generators/manufsol_velocity.py:40-50), simplified.
-      const double st = sin(t), ct = cos(t);
-      const double sx = sin(x), cx = cos(x), sy = sin(y), cy = cos(y), sz = sin(z), cz = cos(z);
-      if (c == 0) return sx * cx * st * st * (2 - 2 * sy * sy - sz * sz) + sx * sz * cy * (ct + 3 * st / f.Re);
-      if (c == 1) return sy * cy * st * st * (2 - 2 * sx * sx - sz * sz) + sy * sz * cx * (ct + 3 * st / f.Re);
-      return sz * cz * st * st * (2 * sx * sx + 2 * sy * sy - 4) + cx * cy * cz * (2 * ct + 6 * st / f.Re);
-    }
+    case MIF_FN_VT_FORCING:
+      return vt_forcing(c, sin(x), cos(x), sin(y), cos(y), sin(z), cos(z), sin(t), cos(t), f.Re);
     case MIF_FN_PT_VELOCITY:
     case MIF_FN_PT_GRAD_P:
       // generators/manufsol_pressure.py:36-38 (grad p equals the velocity)

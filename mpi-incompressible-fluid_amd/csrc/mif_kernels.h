@@ -27,6 +27,10 @@ struct VelStageArgs {
   void* B[3];
   double t, dt;
   FnDesc forcing;
+  // MIF_FN_VT_FORCING from tables of sin/cos of the coordinates (see
+  // launch_trig_tables); null: the forcing is evaluated point by point
+  const double* trig;
+  int tn;
 };
 
 struct Red3 {
@@ -70,6 +74,10 @@ template <typename T>
 void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st);
 template <typename T>
 void launch_vel_stage(int stage, const DGrid& g, const Layout& L, const VelStageArgs& s, hipStream_t st);
+// trig[((a * 2 + s) * 2 + q) * tn + i] = sin (q = 0) or cos (q = 1) of the
+// coordinate of index i on axis a, unshifted (s = 0) or shifted by -h/2
+// (s = 1, the staggered axis of a component), i < tn.
+void launch_trig_tables(const DGrid& g, double* trig, int tn, hipStream_t st);
 template <typename T>
 void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& f, double t, hipStream_t st);
 template <typename T>

@@ -97,7 +97,8 @@ constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;  // plane elements per thread
 #define MIF_STAGE_NS64 3
 #endif
 
-template <typename T, int KIND>
+// FT (velocity-only step): MIF_FN_VT_FORCING read from the trig tables.
+template <typename T, int KIND, bool FT = false>
 __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax,
                                                int kmax, int ntx, int nty, int ntiles) {
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 (plane
@@ -214,7 +215,18 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
         } else {
           // calculate_momentum_rhs_with_forcing_* (include/MomentumEquationForcing.h:10-31)
           const T dt = (T)vs.dt;
-          const T f = rhs + (T)eval_at(g, vs.forcing, c, c, vs.t, i, j, k);
+          double fc;
+          if constexpr (FT) {
+            // component c sits at coordinates shifted on axis c
+            const double* tb = vs.trig;
+            const int tn = vs.tn;
+            auto tab = [&](int a, int q, int idx) { return tb[((a * 2 + (a == c ? 1 : 0)) * 2 + q) * tn + idx]; };
+            fc = vt_forcing(c, tab(0, 0, i), tab(0, 1, i), tab(1, 0, j), tab(1, 1, j), tab(2, 0, k), tab(2, 1, k),
+                            sin(vs.t), cos(vs.t), vs.forcing.Re);
+          } else {
+            fc = eval_at(g, vs.forcing, c, c, vs.t, i, j, k);
+          }
+          const T f = rhs + (T)fc;
           if constexpr (KIND == 4) {
             ((T*)vs.R[c])[o] = f;
             ((T*)vs.B[c])[o] = x + dt * (T)(8.0 / 15.0) * f;
@@ -702,7 +714,7 @@ __global__ void __launch_bounds__(256) k_reduce_final(const Red3* partial, int n
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-template <typename T, int KIND>
+template <typename T, int KIND, bool FT = false>
 static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& s, const VelStageArgs& vs,
                               hipStream_t st) {
   // union of the three components' interiors
@@ -715,7 +727,7 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
   const int ntx = (imax + STX - 1) / STX, nty = (jmax + STY - 1) / STY, ntz = (kmax + SKB - 1) / SKB;
   const int ntiles = ntx * nty * ntz;
-  k_stage<T, KIND><<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
+  k_stage<T, KIND, FT><<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
 }
 
 template <typename T>
@@ -732,12 +744,26 @@ void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s
 template <typename T>
 void launch_vel_stage(int stage, const DGrid& g, const Layout& L, const VelStageArgs& vs, hipStream_t st) {
   StageArgs s{};
+  const bool ft = vs.trig && vs.forcing.kind == MIF_FN_VT_FORCING;
   if (stage == 1)
-    launch_stage_kind<T, 4>(g, L, s, vs, st);
+    ft ? launch_stage_kind<T, 4, true>(g, L, s, vs, st) : launch_stage_kind<T, 4>(g, L, s, vs, st);
   else if (stage == 2)
-    launch_stage_kind<T, 5>(g, L, s, vs, st);
+    ft ? launch_stage_kind<T, 5, true>(g, L, s, vs, st) : launch_stage_kind<T, 5>(g, L, s, vs, st);
   else
-    launch_stage_kind<T, 6>(g, L, s, vs, st);
+    ft ? launch_stage_kind<T, 6, true>(g, L, s, vs, st) : launch_stage_kind<T, 6>(g, L, s, vs, st);
+}
+
+__global__ void __launch_bounds__(256) k_trig_tables(DGrid g, double* __restrict__ trig, int tn) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y / 2, sh = blockIdx.y % 2;
+  if (i >= tn) return;
+  const double x = coord(g, a, sh ? a : 3, i);
+  trig[((a * 2 + sh) * 2 + 0) * tn + i] = sin(x);
+  trig[((a * 2 + sh) * 2 + 1) * tn + i] = cos(x);
+}
+
+void launch_trig_tables(const DGrid& g, double* trig, int tn, hipStream_t st) {
+  k_trig_tables<<<dim3((tn + 255) / 256, 6, 1), 256, 0, st>>>(g, trig, tn);
 }
 
 static void shell_counts(const DGrid& g, int stag, int cnt[3]) {
