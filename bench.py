@@ -247,11 +247,11 @@ def roofline_of(res):
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_channel_256cubed.md")
 CLASS_KERNELS = {
-    "stage": r"k_stage<{T}, [123]>",
-    "fft_x_div": r"k_fft_p2<{T}, \d+, \d+, 0, 0, true>",
-    "fft_x_unpack": r"k_fft_p2<{T}, \d+, \d+, 0, 1, true>",
-    "fft_y": r"k_fft_p2<{T}, \d+, \d+, 1, [01], false>",
-    "fft_z_fused": r"k_fft_p2<{T}, \d+, \d+, 2, 2, false>",
+    "stage": r"k_stage<{T}, [123](, false)?>",
+    "fft_x_div": r"k_fft_p2<{T}, \d+, \d+, 0, 0, true(, \w+)?>",
+    "fft_x_unpack": r"k_fft_p2<{T}, \d+, \d+, 0, 1, true(, \w+)?>",
+    "fft_y": r"k_fft_p2<{T}, \d+, \d+, 1, [01], false(, \w+)?>",
+    "fft_z_fused": r"k_fft_p2<{T}, \d+, \d+, 2, 2, false(, \w+)?>",
     "project": r"k_project<{T}>",
 }
 
@@ -368,6 +368,10 @@ def main():
         if name != "fp64":
             forms[name]["velocity_rel_l2_vs_fp64"] = rel_l2(r["velocity"], ref["velocity"])
             forms[name]["pressure_rel_l2_vs_fp64"] = rel_l2([r["pressure"]], [ref["pressure"]])
+            # the same with each field's mean removed: pressure is defined up
+            # to a constant, which the two forms fix differently
+            forms[name]["pressure_rel_l2_vs_fp64_mean_removed"] = rel_l2([r["pressure"] - r["pressure"].mean()],
+                                                                         [ref["pressure"] - ref["pressure"].mean()])
             forms[name]["speedup_vs_fp64"] = rate / forms["fp64"]["cell_updates_per_s"]
     # headline: the fastest mixed form within 1e-6 relative velocity error of
     # FP64 ("equal error"); FP64 if none qualifies

@@ -238,6 +238,30 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
     }
     if (t == 0) lamy = sc.lam[1][by];
   }
+  // fused x passes, per line: the owner row's first point (divergence), or
+  // the rows the line is written to, owner row and its periodic images in y
+  // and z (unpack); -1: none.  Computed once per workgroup, out of the
+  // per-point loops.
+  __shared__ long prow[AXIS == 0 && FUSED ? 2 * NP : 1][4];
+  if (AXIS == 0 && FUSED && MODE == 0 && t < NL) {
+    const int r = bx * NL + t;
+    prow[t][0] = r < n1 * n2 ? io.L.idx(g.olo[0], r % n1 + g.olo[1], r / n1 + g.olo[2]) : -1;
+  }
+  if (AXIS == 0 && FUSED && MODE == 1 && t < NL) {
+    const int r = bx * NL + t;
+    long rows[4] = {-1, -1, -1, -1};
+    if (r < n1 * n2) {
+      const int N1 = g.N[1], N2 = g.N[2];
+      const int j = r % n1 + g.olo[1], k = r / n1 + g.olo[2];
+      int js[2] = {j, -1}, ks[2] = {k, -1};
+      if (g.periodic[1]) js[1] = j == 1 ? N1 - 1 : (j == N1 - 2 ? 0 : -1);
+      if (g.periodic[2]) ks[1] = k == 1 ? N2 - 1 : (k == N2 - 2 ? 0 : -1);
+      for (int qk = 0; qk < 2; qk++)
+        for (int qj = 0; qj < 2; qj++)
+          if (js[qj] >= 0 && ks[qk] >= 0) rows[2 * qk + qj] = io.L.idx(0, js[qj], ks[qk]);
+    }
+    for (int q = 0; q < 4; q++) prow[t][q] = rows[q];
+  }
   if (t < NL) {
     long b = -1;
     if (AXIS == 0) {
@@ -274,14 +298,13 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
 #pragma unroll
       for (int a = 0; a < CL; a++) {
         const int l = l0 + a;
-        const int r = bx * NL + l;
-        const bool okl = l < NL && r < n1 * n2;
-        const int j = okl ? r % n1 + g.olo[1] : 0, k = okl ? r / n1 + g.olo[2] : 0;
+        const long rb = l < NL ? prow[l][0] : -1;
+        const bool okl = rb >= 0;
 #pragma unroll
         for (int q = 0; q < R0; q++) {
           const int m = t + 256 * q;
           const bool okm = okl && m < n;
-          const long o = okm ? io.L.idx(m + g.olo[0], j, k) : 0;
+          const long o = okm ? rb + m : 0;
           const TV u0 = u[o], u1 = u[o + 1], v0 = v[o], v1 = v[o + PX], w0 = w[o], w1 = w[o + PL];
           const double du_dx = ((double)u1 - (double)u0) * g.one_over_h[0];
           const double dv_dy = ((double)v1 - (double)v0) * g.one_over_h[1];
@@ -432,31 +455,28 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   } else {
     // DCT-I coefficients, or the real lines of an inverse transform
     if (AXIS == 0 && FUSED && MODE == 1) {
+      // all lines' values of a point first, then the stores
       TV* P = (TV*)io.P;
-      const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
-      for (int l = 0; l < NL; l++) {
-        const int r = bx * NL + l;
-        if (r >= n1 * n2) continue;
-        const int j = r % n1 + g.olo[1], k = r / n1 + g.olo[2];
-        // destination rows: the owner row and its periodic images
-        int js[2] = {j, -1}, ks[2] = {k, -1};
-        if (g.periodic[1]) js[1] = j == 1 ? N1 - 1 : (j == N1 - 2 ? 0 : -1);
-        if (g.periodic[2]) ks[1] = k == 1 ? N2 - 1 : (k == N2 - 2 ? 0 : -1);
-        for (int m = t; m < n; m += 256) {
+      const int N0 = g.N[0];
+      for (int m = t; m < n; m += 256) {
+        const int i = m + g.olo[0];
+        const int i2 = g.periodic[0] ? (i == 1 ? N0 - 1 : (i == N0 - 2 ? 0 : -1)) : -1;
+        TV val[NL];
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
           const Cx<T> a = Z[l % NP][pd(m)];
-          const TV val = (TV)(l < NP ? a.re : a.im);
-          const int i = m + g.olo[0];
-          const int i2 = g.periodic[0] ? (i == 1 ? N0 - 1 : (i == N0 - 2 ? 0 : -1)) : -1;
-          for (int qk = 0; qk < 2; qk++) {
-            if (ks[qk] < 0) continue;
-            for (int qj = 0; qj < 2; qj++) {
-              if (js[qj] < 0) continue;
-              const long rowb = io.L.idx(0, js[qj], ks[qk]);
-              P[rowb + i] = val;
-              if (i2 >= 0) P[rowb + i2] = val;
+          val[l] = (TV)(l < NP ? a.re : a.im);
+        }
+#pragma unroll
+        for (int l = 0; l < NL; l++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const long rb = prow[l][q];
+            if (rb >= 0) {
+              P[rb + i] = val[l];
+              if (i2 >= 0) P[rb + i2] = val[l];
             }
           }
-        }
       }
     } else if (AXIS == 0) {
       for (int l = 0; l < NL; l++) {
