@@ -186,6 +186,12 @@ __device__ inline void fft(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
 using p2::Cx;
 using p2::pd;
 
+// LDS bytes one workgroup may hold (gfx950: 160 KB per CU, all of it
+// available to one workgroup)
+#ifndef MIF_LDS_MAX
+#define MIF_LDS_MAX 65536
+#endif
+
 #ifndef MIF_TWL_MAX
 #define MIF_TWL_MAX 65536
 #endif
@@ -204,8 +210,8 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   // bytes, otherwise they are read from the (L1-resident) global table
   constexpr int LP = p2::lds_pitch(L);
   constexpr bool TWL = (NP * LP + L) * sizeof(Cx<T>) <= MIF_TWL_MAX;
-  static_assert(NP * LP * sizeof(Cx<T>) + (L / 2 + 1) * sizeof(double) <= 65536 - (TWL ? L * sizeof(Cx<T>) : 0),
-                "workgroup LDS above 64 KB");
+  static_assert(NP * LP * sizeof(Cx<T>) + (L / 2 + 1) * sizeof(double) <= MIF_LDS_MAX - (TWL ? L * sizeof(Cx<T>) : 0),
+                "workgroup LDS above the limit");
   __shared__ Cx<T> Z[NP][LP];
   __shared__ Cx<T> tws[TWL ? L : 1];
   __shared__ long lbase[2 * NP];
@@ -527,7 +533,7 @@ __global__ void __launch_bounds__(256) k_dct1_y(DGrid g, const T* __restrict__ W
   constexpr int TPL = 256 / NL < 64 ? 256 / NL : 64;
   constexpr int SEG = NH / TPL;
   static_assert(N % 2 == 0 && NH % TPL == 0, "scan segments");
-  static_assert((NP * LP + N + NH + 1) * sizeof(Cx<T>) <= 65536, "workgroup LDS above 64 KB");
+  static_assert((NP * LP + N + NH + 1) * sizeof(Cx<T>) <= MIF_LDS_MAX, "workgroup LDS above the limit");
   constexpr int NI = NP * (NH + 1);         // pre/post items: (pair, j <= N/2)
   constexpr int PER = (NI + 255) / 256;
   __shared__ Cx<T> Z[NP][LP];

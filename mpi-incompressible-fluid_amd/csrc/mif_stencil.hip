@@ -294,40 +294,17 @@ struct BCArgs {
   void* F[3];
   FnDesc f;
   double t;
-  int count[3][3];  // number of shell points per (component, face pair)
+  int count[3][3];  // threads (opposite shell points) per (component, face pair)
 };
 
 // ZERO: homogeneous data (no-slip walls), every face value is 0; the
 // function evaluation code is compiled out, which keeps the kernel small.
+// Value of shell point (i, j, k) of component c: a Dirichlet face value, or
+// the value at its periodic source, which is then an interior point that
+// this pass does not write.
 template <typename T, bool ZERO>
-__global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
-  const int c = blockIdx.y / 3, pair = blockIdx.y % 3;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= a.count[c][pair]) return;
-  int e[3];
-  extents(g, c, e);
-  int i, j, k;
-  // decode the shell point: pair 0 = k planes (all i, j), pair 1 = j planes
-  // (all i, inner k), pair 2 = i planes (inner j, inner k)
-  if (pair == 0) {
-    const int area = e[0] * e[1];
-    const int side = q / area, r = q % area;
-    i = r % e[0];
-    j = r / e[0];
-    k = side ? e[2] - 1 : 0;
-  } else if (pair == 1) {
-    const int area = e[0] * (e[2] - 2);
-    const int side = q / area, r = q % area;
-    i = r % e[0];
-    k = r / e[0] + 1;
-    j = side ? e[1] - 1 : 0;
-  } else {
-    const int area = (e[1] - 2) * (e[2] - 2);
-    const int side = q / area, r = q % area;
-    j = r % (e[1] - 2) + 1;
-    k = r / (e[1] - 2) + 1;
-    i = side ? e[0] - 1 : 0;
-  }
+__device__ inline T shell_value(const DGrid& g, const Layout& L, const BCArgs& a, int c, const int e[3], int i, int j,
+                                int k, const T* F) {
   // periodic ghost source
   int s[3] = {i, j, k};
   for (int ax = 0; ax < 3; ax++) {
@@ -351,16 +328,52 @@ __global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
     if (s[0] == 0) face = 4;
     if (s[0] == e[0] - 1) face = 5;
   }
-  T* F = (T*)a.F[c];
-  T val;
   if (face >= 0) {
     if constexpr (ZERO)
-      val = (T)0;
+      return (T)0;
     else
-      val = (T)face_value(g, a.f, a.t, c, face, s[0], s[1], s[2]);
-  } else
-    val = F[L.idx(s[0], s[1], s[2])];
-  F[L.idx(i, j, k)] = val;
+      return (T)face_value(g, a.f, a.t, c, face, s[0], s[1], s[2]);
+  }
+  return F[L.idx(s[0], s[1], s[2])];
+}
+
+// Each thread owns the two opposite shell points of one (pair, position):
+// pair 0 = the k planes (all i, j), pair 1 = the j planes (all i, inner k),
+// pair 2 = the i planes (inner j, inner k).  On a periodic axis each of the
+// two points is the other's source-side neighbour (ghost 0 and point 1 share
+// a row segment, as do e-2 and ghost e-1), so both reads come first and the
+// two writes then land on lines the same thread has just read.
+template <typename T, bool ZERO>
+__global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
+  const int c = blockIdx.y / 3, pair = blockIdx.y % 3;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.count[c][pair]) return;
+  int e[3];
+  extents(g, c, e);
+  int i, j, k, axis;
+  if (pair == 0) {
+    i = q % e[0];
+    j = q / e[0];
+    k = 0;
+    axis = 2;
+  } else if (pair == 1) {
+    i = q % e[0];
+    k = q / e[0] + 1;
+    j = 0;
+    axis = 1;
+  } else {
+    j = q % (e[1] - 2) + 1;
+    k = q / (e[1] - 2) + 1;
+    i = 0;
+    axis = 0;
+  }
+  int p1[3] = {i, j, k};
+  p1[axis] = e[axis] - 1;
+  T* F = (T*)a.F[c];
+  const T v0 = shell_value<T, ZERO>(g, L, a, c, e, i, j, k, F);
+  const T v1 = shell_value<T, ZERO>(g, L, a, c, e, p1[0], p1[1], p1[2], F);
+  F[L.idx(i, j, k)] = v0;
+  F[L.idx(p1[0], p1[1], p1[2])] = v1;
 }
 
 // Periodic ghost copies of one field (StaggeredTensor::apply_periodic_bc,
@@ -781,7 +794,11 @@ void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& 
   for (int c = 0; c < 3; c++) {
     a.F[c] = F[c];
     shell_counts(g, c, a.count[c]);
-    for (int q = 0; q < 3; q++) mx = max(mx, a.count[c][q]);
+    // one thread per pair of opposite shell points
+    for (int q = 0; q < 3; q++) {
+      a.count[c][q] /= 2;
+      mx = max(mx, a.count[c][q]);
+    }
   }
   a.f = f;
   a.t = t;
