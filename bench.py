@@ -40,6 +40,23 @@ WORDS_PER_LAUNCH = {
     "fft_x_div": 4.0,
     "fft_x_unpack": 2.0,
 }
+# the mixed form with FP64 transforms: bytes per unknown of the classes that
+# touch the FP64 packed buffer (FP32 fields, 8-byte buffer words)
+BYTES_FP64_TRANSFORMS = {
+    "div_rhs": 3 * 4 + 8,
+    "fft_x": 2 * 8,
+    "fft_y": 2 * 8,
+    "fft_z_fused": 2 * 8,
+    "unpack": 8 + 4,
+    "fft_x_div": 3 * 4 + 8,
+    "fft_x_unpack": 8 + 4,
+}
+
+
+def bytes_per_unknown(name: str, res) -> float:
+    if res["esize"] == 4 and res.get("solve_transforms") == "fp64" and name in BYTES_FP64_TRANSFORMS:
+        return BYTES_FP64_TRANSFORMS[name]
+    return WORDS_PER_LAUNCH[name] * res["esize"]
 
 
 def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, keep_fields: bool = True):
@@ -234,7 +251,7 @@ def roofline_of(res):
             # O(N^2) or auxiliary classes (boundary shell, reductions): time only
             table[name] = {"avg_us": round(k["avg_ms"] * 1e3, 2), "launches": k["launches"]}
             continue
-        bytes_per_launch = WORDS_PER_LAUNCH[name] * res["unknowns"] * res["esize"]
+        bytes_per_launch = bytes_per_unknown(name, res) * res["unknowns"]
         gbs = bytes_per_launch / (k["avg_ms"] * 1e-3) / 1e9
         table[name] = {"avg_us": round(k["avg_ms"] * 1e3, 2), "launches": k["launches"],
                        "alg_bytes": int(bytes_per_launch), "GB/s": round(gbs, 1),

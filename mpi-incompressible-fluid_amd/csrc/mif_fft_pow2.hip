@@ -106,6 +106,10 @@ constexpr int nstages(int L) {
   return ilog2(pow2_part(L)) / 3 + (ilog2(pow2_part(L)) % 3 ? 1 : 0) + (has3(L) ? 1 : 0);
 }
 constexpr int ns_of(int L, int s) { return s == 0 ? 1 : ns_of(L, s - 1) * radix_of(L, s - 1); }
+constexpr int min_radix(int L, int s = 0) {
+  return s + 1 >= nstages(L) ? radix_of(L, s)
+                             : (radix_of(L, s) < min_radix(L, s + 1) ? radix_of(L, s) : min_radix(L, s + 1));
+}
 
 // padded LDS index: one pad element every 8
 __device__ constexpr inline int pd(int e) { return e + (e >> 3); }
@@ -113,6 +117,11 @@ __device__ constexpr inline int pd(int e) { return e + (e >> 3); }
 // the lanes of a wave that address the same element of different pairs
 // (load, store and scaling phases) fall in different banks
 constexpr int lds_pitch(int L) { return L + L / 8 + 1; }
+
+// Twiddle entries the stages read: stage S reads tw[k * L / (Ns R)] for
+// k < Ns, all below L / R, so the table needs L / (smallest radix) entries
+// (with MIF_TW_POW; the direct form reads up to L).
+constexpr int tw_count(int L) { return MIF_TW_POW ? L / min_radix(L) : L; }
 
 template <typename T, int L, int NP, int SIGN, int S>
 __device__ inline void stage(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
@@ -209,11 +218,12 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   // twiddles live in LDS when the workgroup stays within MIF_TWL_MAX
   // bytes, otherwise they are read from the (L1-resident) global table
   constexpr int LP = p2::lds_pitch(L);
-  constexpr bool TWL = (NP * LP + L) * sizeof(Cx<T>) <= MIF_TWL_MAX;
-  static_assert(NP * LP * sizeof(Cx<T>) + (L / 2 + 1) * sizeof(double) <= MIF_LDS_MAX - (TWL ? L * sizeof(Cx<T>) : 0),
+  constexpr int TWN = p2::tw_count(L);
+  constexpr bool TWL = (NP * LP + TWN) * sizeof(Cx<T>) <= MIF_TWL_MAX;
+  static_assert(NP * LP * sizeof(Cx<T>) + (L / 2 + 1) * sizeof(double) <= MIF_LDS_MAX - (TWL ? TWN * sizeof(Cx<T>) : 0),
                 "workgroup LDS above the limit");
   __shared__ Cx<T> Z[NP][LP];
-  __shared__ Cx<T> tws[TWL ? L : 1];
+  __shared__ Cx<T> tws[TWL ? TWN : 1];
   __shared__ long lbase[2 * NP];
   const Cx<T>* tw = TWL ? tws : (const Cx<T>*)W;
   constexpr int NL = 2 * NP;
@@ -230,7 +240,7 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   const int n0 = g.npts[0], n1 = g.npts[1], n2 = g.npts[2];
   const long ms = AXIS == 0 ? 1 : (AXIS == 1 ? bp0 : bplane);
   if (TWL)
-    for (int e = t; e < L; e += 256) tws[e] = Cx<T>{W[2 * e], W[2 * e + 1]};
+    for (int e = t; e < TWN; e += 256) tws[e] = Cx<T>{W[2 * e], W[2 * e + 1]};
   // eigenvalues of the scaling step (fused z pass): the z table, this
   // workgroup's x columns and its y row, staged once
   __shared__ double lamz[MODE == 2 ? L / 2 + 1 : 1];
@@ -533,11 +543,12 @@ __global__ void __launch_bounds__(256) k_dct1_y(DGrid g, const T* __restrict__ W
   constexpr int TPL = 256 / NL < 64 ? 256 / NL : 64;
   constexpr int SEG = NH / TPL;
   static_assert(N % 2 == 0 && NH % TPL == 0, "scan segments");
-  static_assert((NP * LP + N + NH + 1) * sizeof(Cx<T>) <= MIF_LDS_MAX, "workgroup LDS above the limit");
+  constexpr int TWN = p2::tw_count(N);
+  static_assert((NP * LP + TWN + NH + 1) * sizeof(Cx<T>) <= MIF_LDS_MAX, "workgroup LDS above the limit");
   constexpr int NI = NP * (NH + 1);         // pre/post items: (pair, j <= N/2)
   constexpr int PER = (NI + 255) / 256;
   __shared__ Cx<T> Z[NP][LP];
-  __shared__ Cx<T> tws[N];       // exp(-2 pi i e / N)
+  __shared__ Cx<T> tws[TWN];     // exp(-2 pi i e / N), e < TWN
   __shared__ Cx<T> cs[NH + 1];   // exp(-i pi j / N)
   __shared__ T s0[NL];
   const int t = threadIdx.x;
@@ -549,7 +560,7 @@ __global__ void __launch_bounds__(256) k_dct1_y(DGrid g, const T* __restrict__ W
   }
   // W is the table of the 2N-point plan, exp(-2 pi i e / 2N): its even
   // entries are the N-point twiddles, its first N/2 + 1 entries cos/sin(pi j/N)
-  for (int e = t; e < N; e += 256) tws[e] = Cx<T>{W[4 * e], W[4 * e + 1]};
+  for (int e = t; e < TWN; e += 256) tws[e] = Cx<T>{W[4 * e], W[4 * e + 1]};
   for (int j = t; j <= NH; j += 256) cs[j] = Cx<T>{W[2 * j], W[2 * j + 1]};
   auto line = [&](int l) -> T* { return (T*)&Z[l % NP][0] + (l < NP ? 0 : LP); };
 
