@@ -1,0 +1,139 @@
+"""The hot path at the full sizes of the measured workloads, through
+properties that hold at any size: agreement with the oracle over a few
+steps, the residual of the exact 7-point operator, the defect correction's
+convergence to the FP64 solve, and the mixed form's velocity error over a
+long run.
+
+- the stand-alone Poisson solve on N_g = 257^3 (periodic x/z, Neumann y),
+  right-hand side ~ N(0,1);
+- the channel 256 x 128 x 256 (N_g = 257, 129, 257; Re = 1000, dt = 5e-4,
+  Poiseuille flow plus a divergence-free perturbation).
+"""
+import numpy as np
+import pytest
+
+import mif_oracle as mo
+from helpers import PI, channel_initial, grids, rel_err, seeded_normal
+from test_oracle_poisson import apply_op, weights
+
+pytestmark = pytest.mark.gpu
+
+mif = pytest.importorskip("mif_amd")
+
+TOL = 1e-11
+
+
+def l2rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def test_poisson_257cubed_residual_and_defect_correction():
+    _case("poisson 257^3 periodic x/z, Neumann y")
+    Ng, per = (257, 257, 257), (1, 0, 1)
+    og, gs = grids(Ng, (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1.0, 1.0, 1, per)
+    o = mo.Oracle(og)
+    b = seeded_normal(tuple(reversed(o.npts)), 20261202)
+    W = weights(o).reshape(b.shape)
+    # the right-hand side projected onto the range of the operator: the
+    # trapezoid-weighted mean removed (the solve does the same)
+    bp = b - (W * b).sum() / W.sum()
+    nb = np.linalg.norm(bp)
+
+    ctx = mif.Context(gs, precision=mif.FP64)
+    ctx.poisson_set_rhs(b)
+    ctx.poisson_run()
+    ref = ctx.poisson_get_solution()
+    ctx.close()
+    res64 = np.linalg.norm(bp - apply_op(o, ref)) / nb
+    print("fp64 residual", res64, flush=True)
+    # FP64 transforms: the residual of the exact operator at rounding level
+    # (measured 8.3e-15)
+    assert res64 < 1e-13
+
+    ctx = mif.Context(gs, precision=mif.MIXED)
+    ctx.poisson_set_rhs(b)
+    errs, res = [], []
+    for sweeps in (0, 1, 2):
+        ctx.set_defect_sweeps(sweeps)
+        ctx.poisson_run()
+        phi = ctx.poisson_get_solution()
+        errs.append(l2rel(phi, ref))
+        res.append(np.linalg.norm(bp - apply_op(o, phi)) / nb)
+    ctx.close()
+    print("mixed errors vs fp64", errs, "residuals", res, flush=True)
+    # FP32 transforms: error bounded by about eps32 * kappa (kappa ~ 1e5
+    # here), each FP64 defect sweep contracts it by a like factor (measured
+    # 2.9e-7, 5.4e-12, 1.8e-13; residuals 4.3e-6, 2.1e-12, 1.4e-15)
+    assert errs[0] < 1e-5
+    assert errs[1] < errs[0] * 1e-2
+    assert errs[2] < 1e-11
+    assert res[2] < 1e-13
+
+
+def _channel(ny=129, steps=50):
+    Ng = (257, ny, 257)
+    return grids(Ng, (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 5e-4 * steps, steps, (1, 0, 1))
+
+
+def test_channel_256x128x256_fp64_matches_oracle():
+    _case("timestep channel 257x129x257 fp64 vs oracle, 3 steps")
+    og, gs = _channel()
+    o = mo.Oracle(og)
+    vel, buf, buf2 = o.velocity(), o.velocity(), o.velocity()
+    p, pb = o.zeros(3), o.zeros(3)
+    v0 = channel_initial(o, 20261202)
+    for c in range(3):
+        vel[c][...] = v0[c]
+    ctx = mif.Context(gs, precision=mif.FP64)
+    gvel, gbuf, gbuf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    gp, gpb = ctx.pressure(), ctx.pressure()
+    gvel.upload(vel)
+    ev = mo.make_fn(mo.FN_ZERO)
+    fn = mif.function(mif.FN_ZERO)
+    for s in range(3):
+        o.timestep(vel, buf, buf2, ev, s * o.dt, p, pb)
+        mif.timestep(gvel, gbuf, gbuf2, fn, s * o.dt, gp, gpb)
+    got = gvel.download()
+    errs = [rel_err(got[c], vel[c]) for c in range(3)]
+    perr = rel_err(gp.download(), p)
+    ctx.close()
+    # condition number of the discrete Laplacian: 4/h^2 summed over the axes
+    # over the smallest nonzero eigenvalue (the periodic x mode 1)
+    h = (2 * PI / 256, 2.0 / 128, PI / 256)
+    kappa = sum(4 / hh ** 2 for hh in h) / (4 * np.sin(PI / 256) ** 2 / h[0] ** 2)
+    print("velocity", errs, "pressure", perr, "kappa", kappa, flush=True)
+    for c in range(3):
+        assert errs[c] < TOL, c
+    # pressure carries the solves' rounding times kappa (see the long
+    # wall-normal-axis case of test_gpu_parity.py)
+    assert perr < TOL * kappa / 1e3
+
+
+def test_channel_256x128x256_mixed_tracks_fp64_over_50_steps():
+    _case("timestep channel 257x129x257 mixed vs fp64, 50 steps")
+    og, gs = _channel()
+    o = mo.Oracle(og)
+    v0 = channel_initial(o, 20261202)
+    out = {}
+    for name, prec in (("fp64", mif.FP64), ("mixed", mif.MIXED)):
+        ctx = mif.Context(gs, precision=prec)
+        if prec == mif.MIXED:
+            ctx.set_defect_sweeps(0)
+        gvel, gbuf, gbuf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+        gp, gpb = ctx.pressure(), ctx.pressure()
+        gvel.upload(v0)
+        fn = mif.function(mif.FN_ZERO)
+        for s in range(50):
+            mif.timestep(gvel, gbuf, gbuf2, fn, s * ctx.dt, gp, gpb)
+        out[name] = [np.asarray(c, dtype=np.float64) for c in gvel.download()]
+        ctx.close()
+    num = sum(np.linalg.norm(a - b) ** 2 for a, b in zip(out["mixed"], out["fp64"]))
+    den = sum(np.linalg.norm(b) ** 2 for b in out["fp64"])
+    err = float(np.sqrt(num / den))
+    print("mixed velocity rel L2 vs fp64 after 50 steps", err, flush=True)
+    assert all(np.isfinite(c).all() for c in out["mixed"])
+    assert err < 1e-6
