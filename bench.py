@@ -355,6 +355,8 @@ def main():
     ap.add_argument("--adjoint-directions", type=int, default=16)
     ap.add_argument("--large", type=int, default=0, metavar="STEPS",
                     help="also run the channel at 768^3 (mixed) and 512^3 (FP64) for STEPS steps")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the --large configurations and print their results (no headline line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build holds the whole domain on one GPU "
@@ -365,6 +367,12 @@ def main():
 
     import mif_amd as mif
     from mif_amd.problems import channel_grid, channel_initial_velocity
+
+    if args.large_only:
+        if args.large <= 0:
+            sys.exit("bench.py: --large-only needs --large STEPS")
+        print(json.dumps({"large": large_leg(args.large)}), flush=True)
+        return
 
     init = channel_initial_velocity(channel_grid(args.n, args.n, args.n))
     forms = {}
@@ -385,10 +393,15 @@ def main():
         if name != "fp64":
             forms[name]["velocity_rel_l2_vs_fp64"] = rel_l2(r["velocity"], ref["velocity"])
             forms[name]["pressure_rel_l2_vs_fp64"] = rel_l2([r["pressure"]], [ref["pressure"]])
-            # the same with each field's mean removed: pressure is defined up
-            # to a constant, which the two forms fix differently
-            forms[name]["pressure_rel_l2_vs_fp64_mean_removed"] = rel_l2([r["pressure"] - r["pressure"].mean()],
-                                                                         [ref["pressure"] - ref["pressure"].mean()])
+            # the same with each field's mean removed (pressure is defined up
+            # to a constant), and the absolute scales: the FP32 velocity's
+            # storage rounding, projected each stage, adds a pressure field of
+            # order eps32 |u| h / dt that does not depend on the solve's
+            # precision; this channel's own pressure is small beside it
+            pm, rm = r["pressure"] - r["pressure"].mean(), ref["pressure"] - ref["pressure"].mean()
+            forms[name]["pressure_rel_l2_vs_fp64_mean_removed"] = rel_l2([pm], [rm])
+            forms[name]["pressure_diff_rms_vs_fp64"] = float(np.sqrt(np.mean((pm - rm) ** 2)))
+            forms[name]["pressure_rms_fp64"] = float(np.sqrt(np.mean(rm ** 2)))
             forms[name]["speedup_vs_fp64"] = rate / forms["fp64"]["cell_updates_per_s"]
     # headline: the fastest mixed form within 1e-6 relative velocity error of
     # FP64 ("equal error"); FP64 if none qualifies
