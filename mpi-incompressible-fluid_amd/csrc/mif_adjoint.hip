@@ -176,10 +176,13 @@ __global__ void __launch_bounds__(256) k_adj_zb(DGrid g, Layout L, AdjVec Y, con
           for (int qc = 0; qc < nopt[2]; qc++) s += term(opts[0][qa], opts[1][qb], opts[2][qc]);
     }
     const long o = L.idx(Q[0], Q[1], Q[2]);
-    const double zov = (double)zo[o];  // unused (and possibly stale) when beta == 0
     const T z = (T)s;
     zb[o] = z;
-    nb[o] = (T)(alpha * (double)z + (beta != 0.0 ? beta * zov : 0.0));
+    // the later stage's z_bar enters only with beta != 0 (stage 2): the
+    // other stages skip its load (a uniform branch)
+    double nbv = alpha * (double)z;
+    if (beta != 0.0) nbv += beta * (double)zo[o];
+    nb[o] = (T)nbv;
     part += (double)z;
   }
   __shared__ double sh[4];

@@ -4,10 +4,12 @@ forward(u0, f) runs `nsteps` full RK3 timesteps (mif_timestep) from the
 initial velocity u0 (three FP64 device tensors in the reference layout,
 shape (sz, sy, sx) per component) with a constant body force f (3,) and
 returns u(T).  It keeps a checkpoint of (u, p) every `checkpoint_every`
-steps.  backward replays each checkpoint interval once, keeping the three
-stencil sources of every step (u^n and the two stage buffers), and runs the
-hand-written adjoint of each step (mif_timestep_adjoint) in reverse.  Memory:
-one (u, p) per checkpoint plus 3 velocity triples per step of one interval.
+steps, and records the three stencil sources of every step of the last
+interval (u^n and the two stage buffers) as it runs them.  backward runs the
+hand-written adjoint of each step (mif_timestep_adjoint) in reverse: the last
+interval from the forward's records, every earlier one after replaying it
+once from its checkpoint.  Memory: one (u, p) per checkpoint plus 3 velocity
+triples per step of one interval.
 
 No counterpart in the reference (it has no adjoint); the forward map is the
 reference's timestep() (src/Timestep.cpp:97-156) with a body force added to
@@ -43,11 +45,14 @@ class Rollout:
         self.rec = [(ctx.velocity(), ctx.velocity(), ctx.velocity()) for _ in range(min(self.k, self.nsteps))]
         self.ubar, self.pbar = ctx.velocity(), ctx.pressure()
         self.shapes = [f.shape for f in self.vel.components]
+        # the records hold the last interval of the latest forward (until a
+        # backward replays an earlier interval into them)
+        self.tail_recorded = False
         # seconds spent in the last backward: replay of the forward steps and
         # the adjoint steps proper (measured only when `timed` is set, which
         # adds a device synchronisation per checkpoint interval)
         self.timed = False
-        self.last_backward = {"recompute_s": 0.0, "adjoint_s": 0.0}
+        self.last_backward = {"recompute_s": 0.0, "adjoint_s": 0.0, "recompute_steps": 0, "adjoint_steps": 0}
 
     # -- device transfers with torch tensors -----------------------------------
     def _load(self, vf: VelocityField, tensors):
@@ -73,16 +78,29 @@ class Rollout:
         timestep(self.vel, self.buf, self.buf2, self.ev, s * self.ctx.dt, self.p, self.pb)
 
     # -- forward / backward -----------------------------------------------------
+    def _record_step(self, s: int, s0: int):
+        # timestep() leaves Y2 and Y3 (after their projections) in its two
+        # buffers, so the recorded triples serve as the buffers; only u^n,
+        # which the step overwrites, is copied
+        un, y2, y3 = self.rec[s - s0]
+        un.copy_from(self.vel)
+        timestep(self.vel, y2, y3, self.ev, s * self.ctx.dt, self.p, self.pb)
+
     def run_forward(self, u0, f):
         self.ctx.set_body_force([float(x) for x in f])
         self._load(self.vel, u0)
         self._zero(self.p)
+        last = (len(self.cp_u) - 1) * self.k
         for s in range(self.nsteps):
             if s % self.k == 0:
                 c = s // self.k
                 self.cp_u[c].copy_from(self.vel)
                 self.cp_p[c].copy_from(self.p)
-            self._step(s)
+            if s >= last:
+                self._record_step(s, last)
+            else:
+                self._step(s)
+        self.tail_recorded = True
         self.ctx.synchronize()
         return self._store(self.vel)
 
@@ -92,6 +110,7 @@ class Rollout:
         adjoint_force_take(self.ctx)  # start from a zero accumulator
         ncp = len(self.cp_u)
         t_rec = t_adj = 0.0
+        n_rec = 0
         clock = time.perf_counter
         for c in reversed(range(ncp)):
             s0 = c * self.k
@@ -99,15 +118,15 @@ class Rollout:
             if self.timed:
                 self.ctx.synchronize()
                 t0 = clock()
-            self.vel.copy_from(self.cp_u[c])
-            self.p.copy_from(self.cp_p[c])
-            for s in range(s0, s1):
-                # timestep() leaves Y2 and Y3 (after their projections) in its
-                # two buffers, so the recorded triples serve as the buffers;
-                # only u^n, which the step overwrites, is copied
-                un, y2, y3 = self.rec[s - s0]
-                un.copy_from(self.vel)
-                timestep(self.vel, y2, y3, self.ev, s * self.ctx.dt, self.p, self.pb)
+            if c == ncp - 1 and self.tail_recorded:
+                pass  # the forward recorded this interval
+            else:
+                self.vel.copy_from(self.cp_u[c])
+                self.p.copy_from(self.cp_p[c])
+                for s in range(s0, s1):
+                    self._record_step(s, s0)
+                n_rec += s1 - s0
+            self.tail_recorded = False
             if self.timed:
                 self.ctx.synchronize()
                 t1 = clock()
@@ -119,7 +138,8 @@ class Rollout:
                 self.ctx.synchronize()
                 t_adj += clock() - t1
         fbar = adjoint_force_take(self.ctx)
-        self.last_backward = {"recompute_s": t_rec, "adjoint_s": t_adj}
+        self.last_backward = {"recompute_s": t_rec, "adjoint_s": t_adj, "recompute_steps": n_rec,
+                              "adjoint_steps": self.nsteps}
         return self._store(self.ubar), fbar
 
     def __call__(self, u0, f):
