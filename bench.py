@@ -270,6 +270,32 @@ def roofline_of(res):
     return roof, table
 
 
+# arrays read and written per launch of a kernel class (one entry per launch
+# kind): the stage kinds 1, 2, 3 and the projection
+STREAM_MIX = {"stage": [(4, 6), (7, 6), (7, 3)], "project": [(5, 4)]}
+
+
+def stream_ceiling(cls: str, res, roof):
+    """What the read/write mix of `cls` streams on this device at all: the
+    mif_stream_probe kernel over arrays of the fields' size, averaged over the
+    class's launch kinds like the class's own timing."""
+    import mif_amd as mif
+    from mif_amd.problems import channel_grid
+
+    if cls not in STREAM_MIX:
+        return None
+    nbytes = res["unknowns"] * res["esize"]
+    ctx = mif.Context(channel_grid(8, 8, 8), precision=mif.FP64, device=0)
+    us = [ctx.stream_probe(r, w, nbytes, reps=10) for r, w in STREAM_MIX[cls]]
+    ctx.close()
+    t = sum(us) / len(us)
+    gbs = roof["alg_bytes_per_launch"] / (t * 1e-6) / 1e9
+    return {"GB/s": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "achieved_over_ceiling": round(roof["achieved"] / gbs, 4),
+            "mixes": ", ".join(f"{r}r+{w}w" for r, w in STREAM_MIX[cls]), "array_bytes": nbytes,
+            "probe_us": [round(x, 1) for x in us]}
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_channel_256cubed.md")
 CLASS_KERNELS = {
     "stage": r"k_stage<{T}, [123](, false)?>",
@@ -424,6 +450,9 @@ def main():
     roof, table = roofline_of(res)
     if roof is not None and args.n == 256:
         roof["traffic"] = measured_traffic(roof["kernel"], res["esize"])
+    if roof is not None:
+        print("[bench] streaming ceiling of the dominant kernel's mix", file=sys.stderr, flush=True)
+        roof["stream_ceiling"] = stream_ceiling(roof["kernel"], res, roof)
     step_words = 108
     step_bytes = step_words * res["unknowns"] * res["esize"]
     step_gbs = step_bytes * args.steps / res["seconds"] / 1e9

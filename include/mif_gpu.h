@@ -241,6 +241,15 @@ int mif_ctx_set_timing(mif_ctx *ctx, int enable);
 int mif_ctx_kernel_time(mif_ctx *ctx, int kernel_class, double *total_ms, uint64_t *launches);
 const char *mif_kernel_class_name(int kernel_class);
 
+/* Measurement aid, no counterpart in the reference: the streaming ceiling of
+ * a read/write mix on this device.  Allocates nread + nwrite device arrays of
+ * `bytes` bytes each (0..8 of each, freed before return), times `reps`
+ * launches of a kernel that reads every byte of the first nread arrays and
+ * writes every byte of the other nwrite (16-byte accesses, grid-stride, on
+ * the context's stream) at 4, 8, 16 and 32 workgroups per CU, and returns
+ * the fastest average launch in microseconds.  Synchronises the stream. */
+int mif_stream_probe(mif_ctx *ctx, int nread, int nwrite, int64_t bytes, int reps, double *us_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1441,6 +1441,55 @@ int mif_ctx_kernel_time(mif_ctx* c, int cls, double* total_ms, uint64_t* launche
   return MIF_OK;
 }
 
+int mif_stream_probe(mif_ctx* c, int nread, int nwrite, int64_t bytes, int reps, double* us_per_launch) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  if (nread < 0 || nwrite < 0 || nread > kStreamMax || nwrite > kStreamMax || nread + nwrite == 0 || reps < 1 ||
+      bytes < 16 || !us_per_launch)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "stream probe: 0..%d reads and writes, bytes >= 16, reps >= 1",
+                kStreamMax);
+  const long n4 = (long)(bytes / 16);
+  StreamArgs a{};
+  void* buf[2 * kStreamMax] = {};
+  const int nb = nread + nwrite;
+  hipError_t err = hipSuccess;
+  for (int q = 0; q < nb && err == hipSuccess; q++) {
+    err = hipMalloc(&buf[q], (size_t)n4 * 16);
+    if (err == hipSuccess) err = hipMemsetAsync(buf[q], 0, (size_t)n4 * 16, c->stream);
+  }
+  int ncu = 0;
+  if (err == hipSuccess) err = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (err == hipSuccess) err = hipEventCreate(&e0);
+  if (err == hipSuccess) err = hipEventCreate(&e1);
+  // the fastest of four grid sizes (4 to 32 workgroups per CU)
+  float best = 0;
+  if (err == hipSuccess) {
+    for (int r = 0; r < nread; r++) a.in[r] = (const float4*)buf[r];
+    for (int w = 0; w < nwrite; w++) a.out[w] = (float4*)buf[nread + w];
+    for (int per_cu = 4; per_cu <= 32 && err == hipSuccess; per_cu *= 2) {
+      const int blocks = per_cu * (ncu > 0 ? ncu : 256);
+      launch_stream(a, nread, nwrite, n4, blocks, c->stream);  // warm-up
+      (void)hipEventRecord(e0, c->stream);
+      for (int i = 0; i < reps; i++) launch_stream(a, nread, nwrite, n4, blocks, c->stream);
+      (void)hipEventRecord(e1, c->stream);
+      err = hipGetLastError();
+      float ms = 0;
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (err == hipSuccess && (best == 0 || ms < best)) best = ms;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(c->stream);
+  for (int q = 0; q < nb; q++) (void)hipFree(buf[q]);
+  if (err != hipSuccess)
+    return fail(c, err == hipErrorOutOfMemory ? MIF_ERR_OUT_OF_MEMORY : MIF_ERR_HIP, "stream probe: %s",
+                hipGetErrorString(err));
+  *us_per_launch = (double)best * 1e3 / reps;
+  return MIF_OK;
+}
+
 const char* mif_kernel_class_name(int cls) {
   static const char* names[MIF_KC_COUNT] = {"stage",  "bc",      "div_rhs", "fft_x",     "fft_y",        "fft_z_fused",
                                             "unpack", "project", "defect",  "fft_x_div", "fft_x_unpack", "adjoint"};

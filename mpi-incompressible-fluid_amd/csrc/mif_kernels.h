@@ -176,4 +176,13 @@ void launch_residual(const DGrid& g, const double* b, const double* P, float* B3
                      hipStream_t st);
 void launch_accumulate(const DGrid& g, double* P, const float* B32, long bp0, long bplane, hipStream_t st);
 
+// Streaming-ceiling probe (mif_probe.hip): nr arrays read and nw arrays
+// written, n4 16-byte elements each.
+constexpr int kStreamMax = 8;
+struct StreamArgs {
+  const float4* in[kStreamMax];
+  float4* out[kStreamMax];
+};
+void launch_stream(const StreamArgs& a, int nr, int nw, long n4, int blocks, hipStream_t st);
+
 }  // namespace mif

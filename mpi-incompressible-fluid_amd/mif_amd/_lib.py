@@ -119,6 +119,7 @@ SIGNATURES = {
     "mif_ctx_set_timing": [_VP, _I],
     "mif_ctx_kernel_time": [_VP, _I, _DP, _U64P],
     "mif_kernel_class_name": [_I],
+    "mif_stream_probe": [_VP, _I, _I, ctypes.c_int64, _I, _DP],
 }
 RESTYPES = {"mif_last_error": ctypes.c_char_p, "mif_kernel_class_name": ctypes.c_char_p}
 

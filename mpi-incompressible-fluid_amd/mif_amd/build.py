@@ -20,7 +20,8 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(PKG_DIR, "csrc")
 OUT = os.path.join(PKG_DIR, "_build")
 LIB = os.path.join(OUT, "libmif_gpu.so")
-SOURCES = ["mif_api.hip", "mif_stencil.hip", "mif_fft.hip", "mif_fft_pow2.hip", "mif_mixed.hip", "mif_adjoint.hip"]
+SOURCES = ["mif_api.hip", "mif_stencil.hip", "mif_fft.hip", "mif_fft_pow2.hip", "mif_mixed.hip", "mif_adjoint.hip",
+           "mif_probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ROOT = os.path.dirname(PKG_DIR)

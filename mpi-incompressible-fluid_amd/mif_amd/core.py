@@ -178,6 +178,15 @@ class Context:
     def set_timing(self, on: bool):
         self.check(self.lib.mif_ctx_set_timing(self.ptr, 1 if on else 0))
 
+    def stream_probe(self, nread: int, nwrite: int, nbytes: int, reps: int = 10) -> float:
+        """Average microseconds of a kernel streaming nread arrays in and
+        nwrite arrays out, nbytes each (mif_stream_probe): the ceiling that
+        read/write mix can reach on this device."""
+        us = ctypes.c_double()
+        self.check(self.lib.mif_stream_probe(self.ptr, int(nread), int(nwrite), int(nbytes), int(reps),
+                                             ctypes.byref(us)))
+        return us.value
+
     def kernel_time(self, cls: int):
         ms = ctypes.c_double()
         n = ctypes.c_uint64()

@@ -31,3 +31,21 @@ def test_field_outliving_its_context():
     rc = lib.mif_field_upload(raw, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     assert rc == 1  # MIF_ERR_INVALID_ARGUMENT
     assert lib.mif_field_destroy(raw) == 0
+
+
+def test_stream_probe():
+    # the streaming-ceiling probe: a copy of 64 MiB moves 128 MiB per launch
+    # at a rate below the device's 8 TB/s peak; arguments out of range are
+    # rejected
+    print("[case] stream probe", flush=True)
+    _, gs = grids((9, 9, 9), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 100.0, 0.01, 1, (1, 0, 1))
+    ctx = mif.Context(gs)
+    nbytes = 64 << 20
+    us = ctx.stream_probe(1, 1, nbytes, reps=5)
+    tbs = 2 * nbytes / (us * 1e-6) / 1e12
+    print("copy", us, "us", tbs, "TB/s", flush=True)
+    assert 0.5 < tbs < 8.5
+    for bad in ((9, 1, nbytes, 5), (0, 0, nbytes, 5), (1, 1, 8, 5), (1, 1, nbytes, 0)):
+        with pytest.raises(mif.MifError):
+            ctx.stream_probe(*bad)
+    ctx.close()
