@@ -38,6 +38,10 @@ struct mif_ctx {
   mif_grid desc{};
   DGrid g{};
   Layout L{};
+  // field storage starts `soff` elements into its allocation, so that point
+  // i = 1 of a row (the first interior point, where the stage kernels' tiles
+  // start) sits on a 128-byte boundary
+  long soff = 0;
   int precision = 0;
   size_t esize = 8;
   int sweeps = 1;
@@ -204,6 +208,17 @@ T* fptr(mif_field* f) {
 }
 
 bool is_fp64(const mif_ctx* c) { return c->precision == MIF_PRECISION_FP64; }
+
+// A field's storage pointer is soff elements into its allocation.
+hipError_t alloc_field_storage(const mif_ctx* c, void** data) {
+  void* raw = nullptr;
+  const hipError_t e = hipMalloc(&raw, (c->L.elems + c->soff) * c->esize);
+  *data = e == hipSuccess ? (char*)raw + c->soff * c->esize : nullptr;
+  return e;
+}
+void free_field_storage(const mif_ctx* c, void* data) {
+  if (data) (void)hipFree((char*)data - c->soff * c->esize);
+}
 
 int check_vel(mif_field* const v[3], mif_ctx** out) {
   if (!v || !v[0] || !v[1] || !v[2]) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null velocity field");
@@ -609,7 +624,7 @@ int adj_workspace(mif_ctx* c) {
   // the component interiors is an invariant the kernels rely on
   for (int q = 0; q < 4; q++)
     for (int cc = 0; cc < 3; cc++) {
-      HIPCHK(c, hipMalloc(&c->adj[q][cc], bytes));
+      HIPCHK(c, alloc_field_storage(c, &c->adj[q][cc]));
       HIPCHK(c, hipMemsetAsync(c->adj[q][cc], 0, bytes, c->stream));
     }
   HIPCHK(c, hipMalloc(&c->adj_partial, 3 * kAdjMaxBlocks * sizeof(double)));
@@ -727,6 +742,11 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
   c->L.SZ = maxz;
   c->L.plane = (long)c->L.PX * c->L.SY;
   c->L.elems = c->L.plane * c->L.SZ;
+#ifndef MIF_FIELD_SHIFT
+#define MIF_FIELD_SHIFT 1
+#endif
+  c->soff = MIF_FIELD_SHIFT ? 128 / (long)c->esize - 1 : 0;
+  c->L.lead = (int)c->soff;
   // kernels decompose flat indices in 32 bits
   if (c->L.elems >= (1L << 31))
     return fail(nullptr, MIF_ERR_UNSUPPORTED, "grid too large: a field would hold %ld padded elements (limit 2^31 - 1)",
@@ -847,7 +867,7 @@ int mif_ctx_destroy(mif_ctx* c) {
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
   for (mif_field* f : c->fields) {
-    (void)hipFree(f->data);
+    free_field_storage(c, f->data);
     f->data = nullptr;
     f->ctx = nullptr;
   }
@@ -857,7 +877,7 @@ int mif_ctx_destroy(mif_ctx* c) {
   (void)hipFree(c->P64);
   (void)hipFree(c->R64);
   for (int q = 0; q < 4; q++)
-    for (int cc = 0; cc < 3; cc++) (void)hipFree(c->adj[q][cc]);
+    for (int cc = 0; cc < 3; cc++) free_field_storage(c, c->adj[q][cc]);
   (void)hipFree(c->adj_partial);
   (void)hipFree(c->adj_fbar);
   (void)hipFree(c->trig);
@@ -921,22 +941,23 @@ int mif_field_create(mif_ctx* c, int stag, mif_field** out) {
   if (stag < 0 || stag > 3) return fail(c, MIF_ERR_INVALID_ARGUMENT, "unknown staggering %d", stag);
   mif_field* f = new (std::nothrow) mif_field{c, stag, nullptr};
   if (!f) return fail(c, MIF_ERR_OUT_OF_MEMORY, "host allocation failed");
-  hipError_t e = hipMalloc(&f->data, c->L.elems * c->esize);
+  hipError_t e = alloc_field_storage(c, &f->data);
   if (e != hipSuccess) {
     delete f;
-    return fail(c, MIF_ERR_OUT_OF_MEMORY, "field allocation of %ld bytes failed", (long)(c->L.elems * c->esize));
+    return fail(c, MIF_ERR_OUT_OF_MEMORY, "field allocation of %ld bytes failed",
+                (long)((c->L.elems + c->soff) * c->esize));
   }
   e = hipMemsetAsync(f->data, 0, c->L.elems * c->esize, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
-    (void)hipFree(f->data);
+    free_field_storage(c, f->data);
     delete f;
     return fail(c, MIF_ERR_HIP, "field clear failed: %s", hipGetErrorString(e));
   }
   try {
     c->fields.insert(f);
   } catch (...) {
-    (void)hipFree(f->data);
+    free_field_storage(c, f->data);
     delete f;
     return fail(c, MIF_ERR_OUT_OF_MEMORY, "host allocation failed");
   }
@@ -949,7 +970,7 @@ int mif_field_destroy(mif_field* f) {
   if (mif_ctx* c = f->ctx) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(f->data);
+    free_field_storage(c, f->data);
     c->fields.erase(f);
   }
   delete f;

@@ -33,6 +33,7 @@ struct Layout {
   int PX, SY, SZ;  // padded pitch, rows per plane, planes
   long plane;      // PX * SY
   long elems;      // PX * SY * SZ
+  int lead;        // elements between the (aligned) allocation start and element 0
   __host__ __device__ inline long idx(int i, int j, int k) const {
     return (long)i + (long)PX * ((long)j + (long)SY * (long)k);
   }

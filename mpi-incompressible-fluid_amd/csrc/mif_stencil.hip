@@ -518,7 +518,10 @@ __global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restric
                                                  double dts, int nqt, int kchunk) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int qt = tile % nqt, kt = tile / nqt;
-  const int q = qt * 256 + threadIdx.x;
+  // slices start at the allocation's aligned origin, L.lead elements
+  // before element 0
+  const int q = qt * 256 + threadIdx.x - L.lead;
+  if (q < 0) return;
   const int i = q % L.PX, j = q / L.PX;
   if (j > g.N[1] || i > g.N[0]) return;
   const int k0 = kt * kchunk, k1 = min(k0 + kchunk, g.N[2] + 1);
@@ -856,7 +859,7 @@ void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, con
   // the interior tests filter), as 256-position slices of the padded plane
   // times chunks of 16 planes
   constexpr int kchunk = 16;
-  const int nqt = (int)((L.plane + 255) / 256);
+  const int nqt = (int)((L.plane + L.lead + 255) / 256);
   const int nkt = (g.N[2] + 1 + kchunk - 1) / kchunk;
   k_project<T><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk);
 }
