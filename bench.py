@@ -171,7 +171,12 @@ def adjoint_leg(n: int, steps: int, k: int, directions: int, eps: float = 1e-6):
             return u0, f, t1 - t0, t2 - t1
 
         run(False)  # warm-up
-        u0, f, tf, tb = run(False)
+        # median of five forward/backward pairs (one repetition alone can
+        # catch a stall of the host or device)
+        reps = [run(False) for _ in range(5)]
+        u0, f = reps[-1][0], reps[-1][1]
+        tf = sorted(r[2] for r in reps)[2]
+        tb = sorted(r[3] for r in reps)[2]
         _, _, _, _ = run(True)
         split = dict(roll.last_backward)
         return ctx, roll, loss_of, target, u0, f, tf, tb, split
