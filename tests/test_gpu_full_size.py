@@ -7,7 +7,9 @@ long run.
 - the stand-alone Poisson solve on N_g = 257^3 (periodic x/z, Neumann y),
   right-hand side ~ N(0,1);
 - the channel 256 x 128 x 256 (N_g = 257, 129, 257; Re = 1000, dt = 5e-4,
-  Poiseuille flow plus a divergence-free perturbation).
+  Poiseuille flow plus a divergence-free perturbation);
+- the Ethier-Steinman run on 64^3 nodes of the unit cube at Re = 1, 10
+  steps, at a stable dt and at the stated dt = 1e-4.
 """
 import numpy as np
 import pytest
@@ -137,3 +139,59 @@ def test_channel_256x128x256_mixed_tracks_fp64_over_50_steps():
     print("mixed velocity rel L2 vs fp64 after 50 steps", err, flush=True)
     assert all(np.isfinite(c).all() for c in out["mixed"])
     assert err < 1e-6
+
+
+def _es_run(dt, steps):
+    # 64^3 nodes on the unit cube, all walls, Re = 1, Ethier-Steinman initial
+    # field and Dirichlet data (the manufactured-solution run of
+    # test/full_test.cpp at the measured configuration's size)
+    Re, N = 1.0, 64
+    og, gs = grids((N, N, N), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), Re, dt * steps, steps, (0, 0, 0))
+    o = mo.Oracle(og)
+    ev = mo.make_fn(mo.FN_ES_VELOCITY, Re=Re)
+    ep = mo.make_fn(mo.FN_ES_PRESSURE, Re=Re)
+    vel, buf, buf2 = o.velocity(), o.velocity(), o.velocity()
+    p, pb = o.zeros(3), o.zeros(3)
+    o.set_velocity(vel, ev, 0.0)
+    o.set(p, 3, ep, 0, 0.0)
+    ctx = mif.Context(gs, precision=mif.FP64)
+    gvel, gbuf, gbuf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    gp, gpb = ctx.pressure(), ctx.pressure()
+    gvel.upload(vel)
+    gp.upload(p)
+    fn = mif.function(mif.FN_ES_VELOCITY, Re=Re)
+    for s in range(steps):
+        o.timestep(vel, buf, buf2, ev, s * o.dt, p, pb)
+        mif.timestep(gvel, gbuf, gbuf2, fn, s * o.dt, gp, gpb)
+    T = steps * o.dt
+    n_ref = o.error_norms_velocity(vel, ev, T)
+    n_gpu = mif.error_norms_velocity(gvel, fn, T)
+    got = gvel.download()
+    errs = [rel_err(got[c], vel[c]) for c in range(3)]
+    ctx.close()
+    return errs, n_gpu, n_ref
+
+
+def test_ethier_steinman_64cubed_re1_stable_dt():
+    # dt = 5e-5: 12 dt / (Re h^2) = 2.38, inside the RK3 stability limit 2.51
+    _case("timestep ES 64^3 walls Re=1 dt=5e-5, 10 steps")
+    errs, n_gpu, n_ref = _es_run(5e-5, 10)
+    print("velocity", errs, "norms", n_gpu, n_ref, flush=True)
+    for c in range(3):
+        assert errs[c] < TOL, c
+    assert np.allclose(n_gpu, n_ref, rtol=1e-9, atol=0)
+
+
+def test_ethier_steinman_64cubed_re1_literal_dt():
+    # the configuration as stated, dt = 1e-4: 12 dt / (Re h^2) = 4.76 exceeds
+    # the RK3 limit, so the highest grid modes grow by |R(-4.76)| = 10.4 per
+    # step (1.5e10 over 10 steps) and rounding differences between any two
+    # evaluation orders grow with them (measured 3.4e-8 relative; the
+    # stable-dt case above agrees to 6e-16)
+    _case("timestep ES 64^3 walls Re=1 dt=1e-4 (beyond the RK3 limit), 10 steps")
+    errs, n_gpu, n_ref = _es_run(1e-4, 10)
+    print("velocity", errs, "norms", n_gpu, n_ref, flush=True)
+    assert all(np.isfinite(n_gpu))
+    for c in range(3):
+        assert errs[c] < 1e-6, c
+    assert np.allclose(n_gpu, n_ref, rtol=1e-6, atol=0)
