@@ -34,6 +34,7 @@ struct Layout {
   long plane;      // PX * SY
   long elems;      // PX * SY * SZ
   int lead;        // elements between the (aligned) allocation start and element 0
+  int nt;          // nonzero: the streaming kernels store their outputs non-temporally
   __host__ __device__ inline long idx(int i, int j, int k) const {
     return (long)i + (long)PX * ((long)j + (long)SY * (long)k);
   }

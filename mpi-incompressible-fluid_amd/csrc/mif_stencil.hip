@@ -85,6 +85,15 @@ __device__ inline bool interior(const DGrid& g, int c, int i, int j, int k) {
 // Outputs may alias pointwise operands (same point, same thread); the
 // stencil source is never written.
 // ---------------------------------------------------------------------------
+// Stores of the stage and projection kernels: non-temporal when NT (the
+// layout's `nt`, set for fields that fit the Infinity Cache).
+template <bool NT, typename T>
+__device__ inline void store_out(T* addr, T val) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(val, addr);
+  else
+    *addr = val;
+}
 constexpr int STX = 64, STY = 8, SHX = STX + 2, SHY = STY + 2, SKB = 16;
 constexpr int STN = STX * STY;                  // threads per workgroup
 constexpr int PLANE_ELEMS = 3 * SHY * SHX;      // one halo plane of u, v, w
@@ -98,7 +107,7 @@ constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;  // plane elements per thread
 #endif
 
 // FT (velocity-only step): MIF_FN_VT_FORCING read from the trig tables.
-template <typename T, int KIND, bool FT = false>
+template <typename T, int KIND, bool FT = false, bool NT = false>
 __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax,
                                                int kmax, int ntx, int nty, int ntiles) {
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 (plane
@@ -203,14 +212,14 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
           // pressure_gradient_{u,v,w} (include/PressureGradient.h:9-25)
           const T pg = (p0 - pm[c]) * (T)g.one_over_h[c];
           if constexpr (KIND == 1) {
-            ((T*)s.O1[c])[o] = x + (T)s.a1 * rhs - (T)s.b * pg;
-            ((T*)s.O2[c])[o] = rhs;
+            store_out<NT>(&((T*)s.O1[c])[o], x + (T)s.a1 * rhs - (T)s.b * pg);
+            store_out<NT>(&((T*)s.O2[c])[o], rhs);
           } else if constexpr (KIND == 2) {
             const T rhs_2_scaled = (T)s.a2 * rhs;
-            ((T*)s.O1[c])[o] = x + (T)s.a1 * av[c] + rhs_2_scaled - (T)s.b * pg;
-            ((T*)s.O2[c])[o] = rhs_2_scaled;
+            store_out<NT>(&((T*)s.O1[c])[o], x + (T)s.a1 * av[c] + rhs_2_scaled - (T)s.b * pg);
+            store_out<NT>(&((T*)s.O2[c])[o], rhs_2_scaled);
           } else {
-            ((T*)s.O1[c])[o] = x + (-av[c]) + (T)s.a2 * rhs - (T)s.b * pg;
+            store_out<NT>(&((T*)s.O1[c])[o], x + (-av[c]) + (T)s.a2 * rhs - (T)s.b * pg);
           }
         } else {
           // calculate_momentum_rhs_with_forcing_* (include/MomentumEquationForcing.h:10-31)
@@ -512,7 +521,7 @@ __global__ void __launch_bounds__(256) k_unpack(DGrid g, Layout L, const S* __re
 // phi at k-1 stays in a register, and the operands of plane k+1 are loaded
 // while plane k is updated.  Loads read always-valid addresses and the
 // stores carry the point tests, so a plane's loads are issued together.
-template <typename T>
+template <typename T, bool NT>
 __global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restrict__ u, T* __restrict__ v,
                                                  T* __restrict__ w, T* __restrict__ p, const T* __restrict__ phi,
                                                  double dts, int nqt, int kchunk) {
@@ -546,11 +555,11 @@ __global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restric
     Pt nxt = cur;
     if (k + 1 < k1) fetch(k + 1, nxt);
     const long o = (long)q + L.plane * k;
-    if (i < g.N[0] && j < g.N[1] && k < g.N[2]) p[o] = cur.pp + cur.f;
+    if (i < g.N[0] && j < g.N[1] && k < g.N[2]) store_out<NT>(&p[o], cur.pp + cur.f);
     const T d[3] = {cur.f - cur.fx, cur.f - cur.fy, cur.f - fz};
 #pragma unroll
     for (int c = 0; c < 3; c++)
-      if (interior(g, c, i, j, k)) F[c][o] = cur.a[c] - d[c] * (T)g.one_over_h[c] * (T)dts;
+      if (interior(g, c, i, j, k)) store_out<NT>(&F[c][o], cur.a[c] - d[c] * (T)g.one_over_h[c] * (T)dts);
     fz = cur.f;
     cur = nxt;
   }
@@ -730,7 +739,7 @@ __global__ void __launch_bounds__(256) k_reduce_final(const Red3* partial, int n
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-template <typename T, int KIND, bool FT = false>
+template <typename T, int KIND, bool FT = false, bool NT = false>
 static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& s, const VelStageArgs& vs,
                               hipStream_t st) {
   // union of the three components' interiors
@@ -743,18 +752,18 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
   const int ntx = (imax + STX - 1) / STX, nty = (jmax + STY - 1) / STY, ntz = (kmax + SKB - 1) / SKB;
   const int ntiles = ntx * nty * ntz;
-  k_stage<T, KIND, FT><<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
+  k_stage<T, KIND, FT, NT><<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
 }
 
 template <typename T>
 void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st) {
   VelStageArgs vs{};
   if (stage == 1)
-    launch_stage_kind<T, 1>(g, L, s, vs, st);
+    L.nt ? launch_stage_kind<T, 1, false, true>(g, L, s, vs, st) : launch_stage_kind<T, 1>(g, L, s, vs, st);
   else if (stage == 2)
-    launch_stage_kind<T, 2>(g, L, s, vs, st);
+    L.nt ? launch_stage_kind<T, 2, false, true>(g, L, s, vs, st) : launch_stage_kind<T, 2>(g, L, s, vs, st);
   else
-    launch_stage_kind<T, 3>(g, L, s, vs, st);
+    L.nt ? launch_stage_kind<T, 3, false, true>(g, L, s, vs, st) : launch_stage_kind<T, 3>(g, L, s, vs, st);
 }
 
 template <typename T>
@@ -861,7 +870,10 @@ void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, con
   constexpr int kchunk = 16;
   const int nqt = (int)((L.plane + L.lead + 255) / 256);
   const int nkt = (g.N[2] + 1 + kchunk - 1) / kchunk;
-  k_project<T><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk);
+  if (L.nt)
+    k_project<T, true><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk);
+  else
+    k_project<T, false><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk);
 }
 
 template <typename T>

@@ -23,9 +23,9 @@ import re
 
 # algorithmic words per pressure unknown per launch (read once + written once)
 ALG_WORDS = [
-    (r"k_stage<[^,]+, 1(, false)?>", 10.0, "stage 1: u,v,w,p -> Y2(3), R1(3)"),
-    (r"k_stage<[^,]+, 2(, false)?>", 13.0, "stage 2: Y2(3),R1(3),p -> Y3(3), S(3)"),
-    (r"k_stage<[^,]+, 3(, false)?>", 10.0, "stage 3: Y3(3),S(3),p -> u(3)"),
+    (r"k_stage<[^,]+, 1(, \w+)*>", 10.0, "stage 1: u,v,w,p -> Y2(3), R1(3)"),
+    (r"k_stage<[^,]+, 2(, \w+)*>", 13.0, "stage 2: Y2(3),R1(3),p -> Y3(3), S(3)"),
+    (r"k_stage<[^,]+, 3(, \w+)*>", 10.0, "stage 3: Y3(3),S(3),p -> u(3)"),
     (r"k_fft_p2<[^>]*, 0, 0, true(, \w+)?>", 4.0, "x forward + divergence: u,v,w -> B"),
     (r"k_fft_p2<[^>]*, 0, 1, true(, \w+)?>", 2.0, "x inverse + unpack: B -> phi"),
     (r"k_fft_p2<[^>]*, 1, [01], false(, \w+)?>", 2.0, "y pass: B -> B"),
