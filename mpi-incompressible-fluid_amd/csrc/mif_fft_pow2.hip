@@ -197,9 +197,10 @@ using p2::pd;
 
 // LDS bytes one workgroup may hold (gfx950: 160 KB per CU, all of it
 // available to one workgroup)
-// Stores of the transform passes into the packed buffer stay ordinary: the
-// next pass reads the buffer back from the Infinity Cache (non-temporal
-// stores there took the 256^3 FP32 y pass from 39 to 43 us).
+// Stores and loads of the transform passes on the packed buffer stay
+// ordinary: the next pass reads the buffer back from the Infinity Cache
+// (non-temporal stores there took the 256^3 FP32 y pass from 39 to 43 us,
+// non-temporal loads to 53 us).
 #define MIF_ST_F(addr, val) (*(addr) = (val))
 // The x inverse pass stores into the pressure field non-temporally at every
 // size (256^3 step with the stage and projection stores non-temporal too:
