@@ -755,16 +755,27 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
   const bool shift = MIF_FIELD_SHIFT && cached;
   c->soff = shift ? 128 / (long)c->esize - 1 : 0;
   c->L.lead = (int)c->soff;
-  // Non-temporal stores in the stage and projection kernels, on the same
-  // condition (the x inverse pass stores non-temporally at every size).  Measured steps, plain -> non-temporal: 256^3 FP64
-  // 3.287 -> 3.110 ms, mixed 1.703 -> 1.632 ms (their output streams exceed
-  // the Infinity Cache, and streaming them past it keeps the next kernel's
-  // lines resident); 768^3 mixed 58.0 -> 59.3 ms (stage kernel +13%), 512^3
-  // FP64 unchanged.
+  // Non-temporal stores (and loads of single-use operands) in the stage and
+  // projection kernels, for fields of kNtMinFieldBytes up to the shift's
+  // limit (the x inverse pass stores non-temporally at every size).
+  // Measured steps, plain -> non-temporal: 256^3 FP64 3.287 -> 3.110 ms, mixed
+  // 1.703 -> 1.632 ms (the stage kernel's 10-13 arrays exceed the Infinity
+  // Cache, and streaming its outputs past it keeps the next kernel's lines
+  // resident); 768^3 mixed 58.0 -> 59.3 ms (stage kernel +13%), 512^3 FP64
+  // unchanged.  Below the lower limit the arrays a stage kernel streams fit
+  // the cache together, so the next kernel finds its operands there:
+  // non-temporal access took the 128^3 FP64 step (18 MB fields) from 0.52 to
+  // 0.62 ms and the mixed one (9 MB) from 0.38 to 0.44 ms.  A 192^3 FP32
+  // field (31 MB) measured the same either way.
 #ifndef MIF_NT_STORES
 #define MIF_NT_STORES 1
 #endif
-  c->L.nt = MIF_NT_STORES && cached ? 1 : 0;
+#ifndef MIF_NT_MIN_FIELD_MB
+#define MIF_NT_MIN_FIELD_MB 32
+#endif
+  constexpr long kNtMinFieldBytes = (long)MIF_NT_MIN_FIELD_MB << 20;
+  const bool streamed = c->L.elems * (long)c->esize > kNtMinFieldBytes;
+  c->L.nt = MIF_NT_STORES && cached && streamed ? 1 : 0;
   // kernels decompose flat indices in 32 bits
   if (c->L.elems >= (1L << 31))
     return fail(nullptr, MIF_ERR_UNSUPPORTED, "grid too large: a field would hold %ld padded elements (limit 2^31 - 1)",
