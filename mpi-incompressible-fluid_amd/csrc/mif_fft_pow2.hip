@@ -2,8 +2,8 @@
 //
 // Same passes and slot order as mif_fft.hip (see there for the algorithm
 // and the reference it follows), specialised at compile time for
-// L = 64, 128, 256, 512, 1024 and 768, 1536 (periodic n = L, Neumann
-// n = L/2 + 1),
+// L = 64, 128, 256, 512, 1024 and 96, 192, 384, 768, 1536 (periodic
+// n = L, Neumann n = L/2 + 1),
 // the lengths of the 2^k- and 3*2^k-interval grids:
 //   * radix-8 Stockham stages (then a radix-4 or -2 stage, and a radix-3
 //     stage for 3 * 2^k), compile-time index arithmetic, data staged in
@@ -782,6 +782,37 @@ __global__ void __launch_bounds__(256) k_dct1_y(DGrid g, const T* __restrict__ W
 #ifndef MIF_NPX_F768
 #define MIF_NPX_F768 2
 #endif
+// lengths 3 * 2^k below 768, as their power-of-two neighbours
+#ifndef MIF_NP_D96
+#define MIF_NP_D96 8
+#endif
+#ifndef MIF_NP_F96
+#define MIF_NP_F96 16
+#endif
+#ifndef MIF_NP_D192
+#define MIF_NP_D192 4
+#endif
+#ifndef MIF_NP_F192
+#define MIF_NP_F192 8
+#endif
+#ifndef MIF_NPX_D192
+#define MIF_NPX_D192 2
+#endif
+#ifndef MIF_NPX_F192
+#define MIF_NPX_F192 4
+#endif
+#ifndef MIF_NP_D384
+#define MIF_NP_D384 2
+#endif
+#ifndef MIF_NP_F384
+#define MIF_NP_F384 4
+#endif
+#ifndef MIF_NPX_D384
+#define MIF_NPX_D384 2
+#endif
+#ifndef MIF_NPX_F384
+#define MIF_NPX_F384 4
+#endif
 
 // Line pairs per workgroup of the half-length Neumann y pass.
 #ifndef MIF_DCT_NP_F512
@@ -886,7 +917,7 @@ static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, c
 
 bool fft_fast_len(int L) {
   switch (L) {
-    case 64: case 128: case 256: case 512: case 1024: case 768: case 1536:
+    case 64: case 128: case 256: case 512: case 1024: case 96: case 192: case 384: case 768: case 1536:
       return true;
     default:
       return false;
@@ -926,6 +957,21 @@ bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, co
         launch_p2<T, 1024, 1, 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
         launch_p2<T, 1024, (d ? MIF_NP_D1024 : MIF_NP_F1024), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 96:
+      launch_p2<T, 96, (d ? MIF_NP_D96 : MIF_NP_F96)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 192:
+      if (axis == 0)
+        launch_p2<T, 192, (d ? MIF_NPX_D192 : MIF_NPX_F192), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      else
+        launch_p2<T, 192, (d ? MIF_NP_D192 : MIF_NP_F192), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      return true;
+    case 384:
+      if (axis == 0)
+        launch_p2<T, 384, (d ? MIF_NPX_D384 : MIF_NPX_F384), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      else
+        launch_p2<T, 384, (d ? MIF_NP_D384 : MIF_NP_F384), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 768:
       if (axis == 0)

@@ -17,7 +17,9 @@ def l2rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("Ng,per", [((65, 33, 65), (1, 0, 1)), ((33, 33, 33), (0, 0, 0)), ((129, 65, 129), (1, 0, 1))])
+@pytest.mark.parametrize("Ng,per", [((65, 33, 65), (1, 0, 1)), ((33, 33, 33), (0, 0, 0)), ((129, 65, 129), (1, 0, 1)),
+                                    # FP32 radix-3 kernels: L = 192 (x, y, z) and 96, 384
+                                    ((193, 97, 193), (1, 0, 1)), ((97, 193, 33), (1, 0, 1))])
 def test_defect_correction_converges(Ng, per):
     print(f"[case] mixed poisson {Ng} {per}", flush=True)
     og, gs = grids(Ng, (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1.0, 1.0, 1, per)

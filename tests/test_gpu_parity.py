@@ -41,6 +41,16 @@ POISSON_CASES = [
     ((513, 9, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),          # x Neumann L = 1024
     ((769, 5, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),          # x Neumann L = 1536
     ((9, 769, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 1, 0)),          # y periodic L = 768
+    # radix-3 kernels of lengths 96, 192, 384
+    ((193, 97, 33), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),     # x periodic L = 192, y Neumann L = 192
+    ((97, 193, 33), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),     # x periodic L = 96, y Neumann L = 384
+    ((385, 9, 17), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), (1, 0, 1)),      # x periodic L = 384
+    ((9, 9, 193), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 1)),          # z periodic L = 192 (fused z)
+    ((9, 9, 97), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 1)),           # z periodic L = 96 (fused z)
+    ((9, 9, 193), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),          # z Neumann L = 384 (fused z)
+    ((193, 5, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),          # x Neumann L = 384
+    ((49, 5, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 0, 0)),           # x Neumann L = 96
+    ((9, 193, 9), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), (0, 1, 0)),          # y periodic L = 192
 ]
 
 
@@ -184,6 +194,18 @@ def test_full_timestep_channel_radix3_fused_path():
     _print_case("timestep channel 769x9x17")
     o, vel, p, gvel, gp = _run_full((769, 9, 17), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 1e-3, 2, (1, 0, 1),
                                     "channel", mo.FN_ZERO, seed=7)
+    for c in range(3):
+        assert rel_err(gvel[c], vel[c]) < TOL, c
+    assert rel_err(gp, p) < TOL
+
+
+@pytest.mark.parametrize("nx", [97, 193, 385])
+def test_full_timestep_channel_radix3_fused_path_small(nx):
+    # x lengths 96, 192, 384 = 3 * 2^k: the fused x passes of the radix-3
+    # kernels below 768
+    _print_case(f"timestep channel {nx}x9x17")
+    o, vel, p, gvel, gp = _run_full((nx, 9, 17), (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 1e-3, 2, (1, 0, 1),
+                                    "channel", mo.FN_ZERO, seed=11)
     for c in range(3):
         assert rel_err(gvel[c], vel[c]) < TOL, c
     assert rel_err(gp, p) < TOL
