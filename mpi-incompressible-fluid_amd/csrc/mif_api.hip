@@ -210,9 +210,13 @@ T* fptr(mif_field* f) {
 bool is_fp64(const mif_ctx* c) { return c->precision == MIF_PRECISION_FP64; }
 
 // A field's storage pointer is soff elements into its allocation.
+// Bytes allocated past a field's last element: the stage kernel's 16-byte
+// row loads that start inside the last row may run up to 12 bytes past it.
+constexpr long kFieldTailBytes = 256;
+
 hipError_t alloc_field_storage(const mif_ctx* c, void** data) {
   void* raw = nullptr;
-  const hipError_t e = hipMalloc(&raw, (c->L.elems + c->soff) * c->esize);
+  const hipError_t e = hipMalloc(&raw, (c->L.elems + c->soff) * c->esize + kFieldTailBytes);
   *data = e == hipSuccess ? (char*)raw + c->soff * c->esize : nullptr;
   return e;
 }
@@ -974,7 +978,7 @@ int mif_field_create(mif_ctx* c, int stag, mif_field** out) {
   if (e != hipSuccess) {
     delete f;
     return fail(c, MIF_ERR_OUT_OF_MEMORY, "field allocation of %ld bytes failed",
-                (long)((c->L.elems + c->soff) * c->esize));
+                (long)((c->L.elems + c->soff) * c->esize + kFieldTailBytes));
   }
   e = hipMemsetAsync(f->data, 0, c->L.elems * c->esize, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -117,12 +117,17 @@ constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;  // plane elements per thread
 #ifndef MIF_STAGE_NS32
 #define MIF_STAGE_NS32 4
 #endif
+#ifndef MIF_STAGE_VEC
+#define MIF_STAGE_VEC 1
+#endif
 #ifndef MIF_STAGE_NS64
 #define MIF_STAGE_NS64 3
 #endif
 
 // FT (velocity-only step): MIF_FN_VT_FORCING read from the trig tables.
-template <typename T, int KIND, bool FT = false, bool NT = false>
+// VEC: halo planes fetched with 16-byte loads (element 1 of every row
+// 16-byte aligned: the layout's storage shift).
+template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = false>
 __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax,
                                                int kmax, int ntx, int nty, int ntiles) {
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 (plane
@@ -139,37 +144,92 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
   const T* X[3];
   for (int f = 0; f < 3; f++) X[f] = (const T*)(KIND <= 3 ? s.X[f] : vs.X[f]);
 
-  // this thread's share of a halo plane: source column and LDS offset
-  const T* src[PQ];
-  int dst[PQ];
-  bool ok[PQ];
+  // this thread's share of a halo plane: source column and LDS offset.
+  // Scalar form: PQ elements.  VEC form: QV 16-byte loads of VW tile-row
+  // elements (row element i0 + VW*iq, 16-byte aligned when element 1 of a
+  // row is) plus at most one halo-column element.
+  constexpr int VW = 16 / (int)sizeof(T);
+  constexpr int RQ = STX / VW;                    // 16-byte loads per tile row
+  constexpr int NIV = 3 * SHY * RQ;               // 16-byte loads per plane
+  constexpr int QV = (NIV + STN - 1) / STN;
+  constexpr int NHV = 3 * SHY * 2;                // halo-column elements per plane
+  static_assert(NHV <= STN, "one halo-column element per thread");
+  constexpr int PR = VEC ? QV * VW + 1 : PQ;      // plane values a thread holds
+  struct alignas(16) Vec {
+    T e[VW];
+  };
+  const T* src[VEC ? QV + 1 : PQ];
+  int dst[VEC ? QV + 1 : PQ];
+  bool ok[VEC ? QV + 1 : PQ];
+  if constexpr (VEC) {
 #pragma unroll
-  for (int r = 0; r < PQ; r++) {
-    const int q = tid + STN * r;
-    const int f = q / (SHY * SHX), rr = q - f * (SHY * SHX);
-    const int jj = rr / SHX, ii = rr - jj * SHX;
-    const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
-    ok[r] = q < PLANE_ELEMS && gi < L.PX && gj < L.SY;
-    src[r] = ok[r] ? X[f] + L.idx(gi, gj, 0) : X[0];
-    dst[r] = q < PLANE_ELEMS ? q : -1;
+    for (int r = 0; r < QV; r++) {
+      const int q = tid + STN * r;
+      const int f = q / (SHY * RQ), rr = q - f * (SHY * RQ);
+      const int jj = rr / RQ, iq = rr - jj * RQ;
+      const int gi = i0 + VW * iq, gj = j0 - 1 + jj;
+      // a load that starts inside the row may run up to VW - 1 elements past
+      // its end (into the next row, or the allocation's tail); those lanes
+      // feed no output point
+      ok[r] = q < NIV && gi < L.PX && gj < L.SY;
+      src[r] = ok[r] ? X[f] + L.idx(gi, gj, 0) : X[0] + 1;
+      dst[r] = q < NIV ? (f * SHY + jj) * SHX + 1 + VW * iq : -1;
+    }
+    const int f = tid / (SHY * 2), rr = tid - f * (SHY * 2);
+    const int jj = rr >> 1, side = rr & 1;
+    const int gi = side ? i0 + STX : i0 - 1, gj = j0 - 1 + jj;
+    ok[QV] = tid < NHV && gi < L.PX && gj < L.SY;
+    src[QV] = ok[QV] ? X[f] + L.idx(gi, gj, 0) : X[0];
+    dst[QV] = tid < NHV ? (f * SHY + jj) * SHX + (side ? SHX - 1 : 0) : -1;
+  } else {
+#pragma unroll
+    for (int r = 0; r < PQ; r++) {
+      const int q = tid + STN * r;
+      const int f = q / (SHY * SHX), rr = q - f * (SHY * SHX);
+      const int jj = rr / SHX, ii = rr - jj * SHX;
+      const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+      ok[r] = q < PLANE_ELEMS && gi < L.PX && gj < L.SY;
+      src[r] = ok[r] ? X[f] + L.idx(gi, gj, 0) : X[0];
+      dst[r] = q < PLANE_ELEMS ? q : -1;
+    }
   }
   // every plane read lies in 0 .. kmax + 1 <= SZ - 1, so the load address is
   // valid for every lane and the loads need no branch
-  auto fetch = [&](int k, T (&v)[PQ]) {
+  auto fetch = [&](int k, T (&v)[PR]) {
+    if constexpr (VEC) {
 #pragma unroll
-    for (int r = 0; r < PQ; r++) {
-      const T x = src[r][(long)k * L.plane];
-      v[r] = ok[r] ? x : (T)0;
+      for (int r = 0; r < QV; r++) {
+        const Vec x = *(const Vec*)(src[r] + (long)k * L.plane);
+#pragma unroll
+        for (int e = 0; e < VW; e++) v[r * VW + e] = ok[r] ? x.e[e] : (T)0;
+      }
+      const T h = src[QV][(long)k * L.plane];
+      v[QV * VW] = ok[QV] ? h : (T)0;
+    } else {
+#pragma unroll
+      for (int r = 0; r < PQ; r++) {
+        const T x = src[r][(long)k * L.plane];
+        v[r] = ok[r] ? x : (T)0;
+      }
     }
   };
-  auto commit = [&](int slot, const T (&v)[PQ]) {
+  auto commit = [&](int slot, const T (&v)[PR]) {
     T* base = &pl[slot][0][0][0];
+    if constexpr (VEC) {
 #pragma unroll
-    for (int r = 0; r < PQ; r++)
-      if (dst[r] >= 0) base[dst[r]] = v[r];
+      for (int r = 0; r < QV; r++)
+        if (dst[r] >= 0)
+#pragma unroll
+          for (int e = 0; e < VW; e++) base[dst[r] + e] = v[r * VW + e];
+      if (dst[QV] >= 0) base[dst[QV]] = v[QV * VW];
+    } else {
+#pragma unroll
+      for (int r = 0; r < PQ; r++)
+        if (dst[r] >= 0) base[dst[r]] = v[r];
+    }
   };
   {
-    T a[PQ], b[PQ], c[PQ];
+    T a[PR], b[PR], c[PR];
     fetch(k0 - 1, a);
     fetch(k0, b);
     fetch(k0 + 1, c);
@@ -177,7 +237,7 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
     commit(k0 % NS, b);
     commit((k0 + 1) % NS, c);
   }
-  T pre[PQ];
+  T pre[PR];
   if (k0 + 2 <= k1) fetch(k0 + 2, pre);
   __syncthreads();
 
@@ -767,6 +827,20 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
   const int ntx = (imax + STX - 1) / STX, nty = (jmax + STY - 1) / STY, ntz = (kmax + SKB - 1) / SKB;
   const int ntiles = ntx * nty * ntz;
+  // the 16-byte loads need element 1 of every row on a 16-byte boundary:
+  // the storage shift puts it on a 128-byte one, and rows are 64-byte
+  // multiples.  Measured against 4- and 8-byte loads (step times): 256^3
+  // FP64 3.000 -> 2.965 ms (stage kernel 362 -> 350 us), 192^3 FP64 1.400 ->
+  // 1.359 ms, 128^3 FP64 0.523 -> 0.509 ms, 192^3 mixed 0.787 -> 0.775 ms;
+  // the 256^3 mixed stage kernel (non-temporal stores) 181 -> 185 us, so
+  // FP32 with non-temporal stores keeps the scalar loads.
+  if constexpr (MIF_STAGE_VEC && KIND <= 3 && (sizeof(T) == 8 || !NT)) {
+    if (L.lead * (long)sizeof(T) == 128 - (long)sizeof(T) && (L.PX * sizeof(T)) % 16 == 0) {
+      k_stage<T, KIND, FT, NT, true>
+          <<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
+      return;
+    }
+  }
   k_stage<T, KIND, FT, NT><<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
 }
 
