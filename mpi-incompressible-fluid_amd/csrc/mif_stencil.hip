@@ -130,9 +130,9 @@ constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;  // plane elements per thread
 template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = false>
 __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax,
                                                int kmax, int ntx, int nty, int ntiles) {
-  // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 (plane
-  // k+2 is committed into the fourth slot while plane k is computed; one
-  // barrier per plane)
+  // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 and
+  // more (plane k+2 is committed into a free slot while plane k is
+  // computed; one barrier per plane)
   constexpr int NS = sizeof(T) == 4 ? MIF_STAGE_NS32 : MIF_STAGE_NS64;
   __shared__ T pl[NS][3][SHY][SHX];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * STX + tx;
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
 
   const long doff[3] = {1, L.PX, L.plane};
   for (int k = k0; k < k1; k++) {
-    if constexpr (NS == 4) {
+    if constexpr (NS >= 4) {
       // the slot of plane k-2, which no thread reads any more
       if (k + 2 <= k1) commit((k + 2) % NS, pre);
       if (k + 3 <= k1) fetch(k + 3, pre);
@@ -832,8 +832,10 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   // multiples.  Measured against 4- and 8-byte loads (step times): 256^3
   // FP64 3.000 -> 2.965 ms (stage kernel 362 -> 350 us), 192^3 FP64 1.400 ->
   // 1.359 ms, 128^3 FP64 0.523 -> 0.509 ms, 192^3 mixed 0.787 -> 0.775 ms;
-  // the 256^3 mixed stage kernel (non-temporal stores) 181 -> 185 us, so
-  // FP32 with non-temporal stores keeps the scalar loads.
+  // the 256^3 mixed stage kernel (non-temporal stores) 181 -> 185 us (182
+  // with the LDS ring padded to six slots, i.e. three workgroups per CU as
+  // with scalar loads), so FP32 with non-temporal stores keeps the scalar
+  // loads.
   if constexpr (MIF_STAGE_VEC && KIND <= 3 && (sizeof(T) == 8 || !NT)) {
     if (L.lead * (long)sizeof(T) == 128 - (long)sizeof(T) && (L.PX * sizeof(T)) % 16 == 0) {
       k_stage<T, KIND, FT, NT, true>
