@@ -195,3 +195,55 @@ def test_ethier_steinman_64cubed_re1_literal_dt():
     for c in range(3):
         assert errs[c] < 1e-6, c
     assert np.allclose(n_gpu, n_ref, rtol=1e-6, atol=0)
+
+
+def test_channel_512x256x512_unshifted_layout_matches_oracle():
+    # fields above 256 MB (FP64 562 MB, FP32 281 MB) keep element 0 at the
+    # allocation start: the stage kernel's 4/8-byte halo loads, ordinary
+    # stores and the projection's unshifted slices, as in the 512^3 FP64 and
+    # 768^3 mixed workloads
+    _case("timestep channel 513x257x513 fp64 and mixed vs oracle, 2 steps (unshifted layout)")
+    Ng, steps = (513, 257, 513), 2
+    og, gs = grids(Ng, (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1e3, 5e-4 * steps, steps, (1, 0, 1))
+    o = mo.Oracle(og)
+    vel, buf, buf2 = o.velocity(), o.velocity(), o.velocity()
+    p, pb = o.zeros(3), o.zeros(3)
+    v0 = channel_initial(o, 20261203)
+    for c in range(3):
+        vel[c][...] = v0[c]
+    ev = mo.make_fn(mo.FN_ZERO)
+    for s in range(steps):
+        o.timestep(vel, buf, buf2, ev, s * o.dt, p, pb)
+    del buf, buf2, pb
+    h = (2 * PI / 512, 2.0 / 256, PI / 512)
+    kappa = sum(4 / hh ** 2 for hh in h) / (4 * np.sin(PI / 512) ** 2 / h[0] ** 2)
+    for name, prec in (("fp64", mif.FP64), ("mixed", mif.MIXED)):
+        ctx = mif.Context(gs, precision=prec)
+        if prec == mif.MIXED:
+            ctx.set_defect_sweeps(0)
+        gvel, gbuf, gbuf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+        gp, gpb = ctx.pressure(), ctx.pressure()
+        gvel.upload(v0)
+        fn = mif.function(mif.FN_ZERO)
+        for s in range(steps):
+            mif.timestep(gvel, gbuf, gbuf2, fn, s * o.dt, gp, gpb)
+        got = [np.asarray(g, dtype=np.float64) for g in gvel.download()]
+        errs = [rel_err(got[c], vel[c]) for c in range(3)]
+        l2 = float(np.sqrt(sum(np.linalg.norm(got[c] - vel[c]) ** 2 for c in range(3)) /
+                           sum(np.linalg.norm(vel[c]) ** 2 for c in range(3))))
+        perr = rel_err(np.asarray(gp.download(), dtype=np.float64), p)
+        ctx.close()
+        del got, gvel, gbuf, gbuf2, gp, gpb
+        print(name, "velocity", errs, "rel L2", l2, "pressure", perr, "kappa", kappa, flush=True)
+        if prec == mif.FP64:
+            for c in range(3):
+                assert errs[c] < TOL, c
+            assert perr < TOL * kappa / 1e3
+        else:
+            # FP32 storage and transforms (kappa ~ 2e5, below the FP64-transform
+            # switch): the velocity's relative L2 distance over the three
+            # components within the mixed form's 1e-6 bar (per component, the
+            # small perturbation components v and w carry the FP32 rounding of
+            # the Poiseuille u: max-norm 2e-5 of their own size)
+            assert l2 < 1e-6
+            assert all(np.isfinite(errs)) and np.isfinite(perr)
