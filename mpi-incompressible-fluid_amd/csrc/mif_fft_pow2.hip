@@ -113,10 +113,16 @@ constexpr int min_radix(int L, int s = 0) {
 
 // padded LDS index: one pad element every 8
 __device__ constexpr inline int pd(int e) { return e + (e >> 3); }
-// LDS pitch of one line pair: the padded length plus one element, so that
-// the lanes of a wave that address the same element of different pairs
-// (load, store and scaling phases) fall in different banks
-constexpr int lds_pitch(int L) { return L + L / 8 + 1; }
+// LDS pitch of one line pair: the padded length plus two elements, so that
+// the lanes of a wave that address neighbouring elements of different pairs
+// (load, store and scaling phases of the strided passes: lanes run over the
+// lines first, then over the elements) fall in different banks.  Measured
+// at 256^3 against one element: FP32 y pass 38.4 -> 37.0 us, mixed step
+// 1.576 -> 1.561 ms (four elements: the same; eight: the z pass slower)
+#ifndef MIF_LP_EXTRA
+#define MIF_LP_EXTRA 2
+#endif
+constexpr int lds_pitch(int L) { return L + L / 8 + MIF_LP_EXTRA; }
 
 // Twiddle entries the stages read: stage S reads tw[k * L / (Ns R)] for
 // k < Ns, all below L / R, so the table needs L / (smallest radix) entries
