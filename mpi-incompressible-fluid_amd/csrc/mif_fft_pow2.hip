@@ -111,7 +111,9 @@ constexpr int min_radix(int L, int s = 0) {
                              : (radix_of(L, s) < min_radix(L, s + 1) ? radix_of(L, s) : min_radix(L, s + 1));
 }
 
-// padded LDS index: one pad element every 8
+// padded LDS index: one pad element every 8 (one every 16, which a bank
+// model of the radix stages rates conflict-free, measured no faster: 256^3
+// mixed step 1.574 -> 1.582 ms)
 __device__ constexpr inline int pd(int e) { return e + (e >> 3); }
 // LDS pitch of one line pair: the padded length plus two elements, so that
 // the lanes of a wave that address neighbouring elements of different pairs
