@@ -405,7 +405,7 @@ def main():
                     help="run only the --large configurations and print their results (no headline line)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build holds the whole domain on one GPU "
+        print(f"bench.py: --gpus {args.gpus} is not supported: the solver holds the whole domain on one GPU "
               f"(no ranks, no collectives); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

@@ -27,7 +27,7 @@ typedef enum mif_status {
   MIF_ERR_HIP = 2,              /* a HIP runtime call failed                 */
   MIF_ERR_OUT_OF_MEMORY = 3,    /* device allocation failed                  */
   MIF_ERR_NON_FINITE = 4,       /* a NaN/Inf was detected in a reduction     */
-  MIF_ERR_UNSUPPORTED = 5,      /* valid request this build does not handle  */
+  MIF_ERR_UNSUPPORTED = 5,      /* valid request the library does not handle */
   MIF_ERR_NO_DEVICE = 6         /* no HIP device is visible                  */
 } mif_status;
 

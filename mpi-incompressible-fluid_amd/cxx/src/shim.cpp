@@ -117,7 +117,7 @@ Constants::Constants(size_t Nx_global, size_t Ny_global, size_t Nz_global, Real 
       prev_proc_z(-1),
       next_proc_z(-1) {
   if (Py != 1 || Pz != 1 || rank != 0)
-    throw std::invalid_argument("this build holds the whole domain on one GPU: Py = Pz = 1 and rank 0 are required");
+    throw std::invalid_argument("the solver holds the whole domain on one GPU: Py = Pz = 1 and rank 0 are required");
   const mif_grid g = grid_of(Nx_global, Ny_global, Nz_global, x_size, y_size_global, z_size_global, min_x_global,
                              min_y_global, min_z_global, Re, final_time, num_time_steps, periodic_bc);
   check_status(mif_ctx_create(&g, precision, device, &ctx_), nullptr, "mif_ctx_create");
