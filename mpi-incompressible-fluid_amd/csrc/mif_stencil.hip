@@ -63,7 +63,8 @@ __device__ inline bool interior(const DGrid& g, int c, int i, int j, int k) {
 // RK3 stage kernels (src/Timestep.cpp:10-54 and src/TimestepVelocity.cpp:
 // 20-51), all three components in one pass.
 //
-// A 64 x 8 workgroup owns an (i, j) tile and marches along k.  The planes
+// A 64 x 8 (FP64) or 128 x 4 (FP32) workgroup owns an (i, j) tile and
+// marches along k.  The planes
 // k-1, k, k+1 of the stencil source (u, v, w with a one-point halo in i and
 // j) live in an LDS ring buffer, so each source value is fetched from
 // memory about once; every tap of the 15-point stencils is an LDS read.
@@ -109,10 +110,33 @@ __device__ inline void store_out(T* addr, T val) {
   else
     *addr = val;
 }
-constexpr int STX = 64, STY = 8, SHX = STX + 2, SHY = STY + 2, SKB = 16;
-constexpr int STN = STX * STY;                  // threads per workgroup
-constexpr int PLANE_ELEMS = 3 * SHY * SHX;      // one halo plane of u, v, w
-constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;  // plane elements per thread
+// Stage-kernel tile: STX x STY points of a plane per 512-thread workgroup,
+// per element type.  Measured at 256^3 against 64 x 8: 128 x 4 took the FP32
+// stage kernel from 182.5 to 177.7 us (mixed step 1.562 -> 1.552 ms) and the
+// FP64 one from 335 to 345 us.
+#ifndef MIF_STX32
+#define MIF_STX32 128
+#endif
+#ifndef MIF_STY32
+#define MIF_STY32 4
+#endif
+#ifndef MIF_STX64
+#define MIF_STX64 64
+#endif
+#ifndef MIF_STY64
+#define MIF_STY64 8
+#endif
+template <typename T>
+struct StageTile {
+  static constexpr int STX = sizeof(T) == 4 ? MIF_STX32 : MIF_STX64;
+  static constexpr int STY = sizeof(T) == 4 ? MIF_STY32 : MIF_STY64;
+  static constexpr int SHX = STX + 2, SHY = STY + 2;
+  static constexpr int STN = STX * STY;                      // threads per workgroup
+  static constexpr int PLANE_ELEMS = 3 * SHY * SHX;          // one halo plane of u, v, w
+  static constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;   // plane elements per thread
+  static_assert(STN == 512, "the stage kernel's launch bounds");
+};
+constexpr int SKB = 16;  // planes per workgroup
 
 #ifndef MIF_STAGE_NS32
 #define MIF_STAGE_NS32 4
@@ -141,6 +165,9 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
   // loaded (FP64: 256^3 stage kernel 348 -> 341 us, step 2.955 -> 2.928 ms;
   // FP32: 181 -> 183 us, so FP32 keeps the load)
   constexpr bool PCARRY = MIF_STAGE_PCARRY && sizeof(T) == 8;
+  using TT = StageTile<T>;
+  constexpr int STX = TT::STX, STY = TT::STY, SHX = TT::SHX, SHY = TT::SHY, STN = TT::STN;
+  constexpr int PLANE_ELEMS = TT::PLANE_ELEMS, PQ = TT::PQ;
   __shared__ T pl[NS][3][SHY][SHX];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * STX + tx;
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -842,6 +869,7 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   const int imax = max(eu[0], max(ev[0], ew[0])) - 2;
   const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
   const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
+  constexpr int STX = StageTile<T>::STX, STY = StageTile<T>::STY;
   const int ntx = (imax + STX - 1) / STX, nty = (jmax + STY - 1) / STY, ntz = (kmax + SKB - 1) / SKB;
   const int ntiles = ntx * nty * ntz;
   // the 16-byte loads need element 1 of every row on a 16-byte boundary:
