@@ -163,7 +163,8 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
   constexpr int NS = sizeof(T) == 4 ? MIF_STAGE_NS32 : MIF_STAGE_NS64;
   // p at plane k-1 carried in a register from plane to plane instead of
   // loaded (FP64: 256^3 stage kernel 348 -> 341 us, step 2.955 -> 2.928 ms;
-  // FP32: 181 -> 183 us, so FP32 keeps the load)
+  // FP32: 181 -> 183 us on 64 x 8 tiles, 176 -> 179 us on 128 x 4, so FP32
+  // keeps the load)
   constexpr bool PCARRY = MIF_STAGE_PCARRY && sizeof(T) == 8;
   using TT = StageTile<T>;
   constexpr int STX = TT::STX, STY = TT::STY, SHX = TT::SHX, SHY = TT::SHY, STN = TT::STN;
@@ -880,7 +881,7 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   // the 256^3 mixed stage kernel (non-temporal stores) 181 -> 185 us (182
   // with the LDS ring padded to six slots, i.e. three workgroups per CU as
   // with scalar loads), so FP32 with non-temporal stores keeps the scalar
-  // loads.
+  // loads (on the FP32 128 x 4 tiles: 176.2 -> 175.9 us, no gain either).
   if constexpr (MIF_STAGE_VEC && KIND <= 3 && (sizeof(T) == 8 || !NT)) {
     if (L.lead * (long)sizeof(T) == 128 - (long)sizeof(T) && (L.PX * sizeof(T)) % 16 == 0) {
       k_stage<T, KIND, FT, NT, true>
