@@ -136,7 +136,9 @@ struct StageTile {
   static constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;   // plane elements per thread
   static_assert(STN == 512, "the stage kernel's launch bounds");
 };
-constexpr int SKB = 16;  // planes per workgroup
+// planes per workgroup (FP32 stage kernel on 128 x 4 tiles at 256^3: 8
+// planes 180.4 us, 16 176.5 us, 24 178.0 us, 32 192.5 us)
+constexpr int SKB = 16;
 
 #ifndef MIF_STAGE_NS32
 #define MIF_STAGE_NS32 4
