@@ -19,6 +19,7 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib as L
+from .restart import format_text_dump
 
 
 class MifError(RuntimeError):
@@ -244,12 +245,8 @@ class Field:
     def dump(self, path: str):
         """Text dump in the reference's format: the dimensions line, then the
         values in storage order (x fastest) with 6 significant digits."""
-        sx = self.shape[::-1]
-        vals = self.download().ravel()
         with open(path, "w") as f:
-            f.write("Tensor (row major). Dimensions: [ %d %d %d ]\n[ " % sx)
-            f.write("".join("%g " % v for v in vals))
-            f.write("]\n")
+            f.write(format_text_dump(self.download()))
 
     def load(self, path: str):
         """Values from a text dump or a plain whitespace-separated list, in

@@ -762,8 +762,7 @@ static void apply_nhn(const orc_grid *g, double *rhs, const mif_function *gp) {
  * (src/PressureEquation.cpp:266-286): rhs = div/dt on owner points
  * (:59-61), optional Neumann data, pack (src/PressureTensor.cpp:16-19), solve,
  * unpack and periodic copies (src/PressureTensor.cpp:21-34). */
-void orc_solve_pressure(const orc_grid *g, double *p, double *const vel[3], const mif_function *gp,
-                        double dt) {
+void orc_pressure_rhs(const orc_grid *g, double *p, double *const vel[3], const mif_function *gp, double dt) {
   long e[3], lo[3], hi[3];
   orc_extents(g, 3, e);
   owner_range(g, lo, hi);
@@ -772,6 +771,14 @@ void orc_solve_pressure(const orc_grid *g, double *p, double *const vel[3], cons
     for (long j = lo[1]; j < hi[1]; j++)
       for (long i = lo[0]; i < hi[0]; i++) AT(p, e, i, j, k) = divergence(g, vel, i, j, k) / dt;
   if (gp) apply_nhn(g, p, gp);
+}
+
+void orc_solve_pressure(const orc_grid *g, double *p, double *const vel[3], const mif_function *gp,
+                        double dt) {
+  long e[3], lo[3], hi[3];
+  orc_extents(g, 3, e);
+  owner_range(g, lo, hi);
+  orc_pressure_rhs(g, p, vel, gp, dt);
   const long *n = g->npts;
   double *b = malloc(sizeof(double) * (size_t)(n[0] * n[1] * n[2]));
 #pragma omp parallel for schedule(static)
@@ -814,6 +821,29 @@ static void add_pressure(const orc_grid *g, double *p, const double *phi) {
   for (long q = 0; q < n; q++) p[q] += phi[q];
 }
 
+/* The boundary function timestep_nhn hands to the pressure solve of stage s:
+ * PRESSURE_EQUATION_true(velocity, dt, new_time, prev_time) passes
+ * exact_pressure_gradient.get_difference_over_time(new_time, prev_time)
+ * (src/Timestep.cpp:89-95) with (time_1, t_n), (time_2, time_1) and
+ * (final_time, time_2) (:110, :122, :134), and get_difference_over_time(a, b)
+ * returns f(b) - f(a) (src/VectorFunction.cpp:53-61): the function is
+ * gradp(prev_time) - gradp(new_time). */
+void orc_nhn_stage_function(const orc_grid *g, const mif_function *gradp, double t_n, int stage,
+                            mif_function *out) {
+  const double dt = g->dt;
+  const double time_1 = t_n + 64.0 / 120.0 * dt;
+  const double time_2 = time_1 + 16.0 / 120.0 * dt;
+  const double final_time = t_n + dt;
+  memset(out, 0, sizeof(*out));
+  out->kind = MIF_FN_TIME_DIFFERENCE;
+  out->base = gradp->kind;
+  out->Re = gradp->Re;
+  memcpy(out->c, gradp->c, sizeof(out->c));
+  /* value = base(t1) - base(t0) */
+  out->t0 = stage == 1 ? time_1 : (stage == 2 ? time_2 : final_time);
+  out->t1 = stage == 1 ? t_n : (stage == 2 ? time_1 : time_2);
+}
+
 void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3], double *const buf2[3],
                   const mif_function *ev, const mif_function *gradp, double t_n, double *p,
                   double *p_buf) {
@@ -825,12 +855,6 @@ void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3],
   const double dt_3 = 40.0 / 120.0 * dt;
   const double final_time = t_n + dt;
   mif_function gdiff;
-  if (gradp) {
-    gdiff.kind = MIF_FN_TIME_DIFFERENCE;
-    gdiff.base = gradp->kind;
-    gdiff.Re = gradp->Re;
-    memcpy(gdiff.c, gradp->c, sizeof(gdiff.c));
-  }
 
   /* Stage 1: COMPUTE_COMPONENT_Y2 (src/Timestep.cpp:10-22) */
   {
@@ -850,12 +874,7 @@ void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3],
     }
   }
   orc_apply_bc(g, buf[0], buf[1], buf[2], ev, time_1);
-  /* get_difference_over_time(new_time, prev_time) returns f(prev) - f(new)
-   * (src/Timestep.cpp:89-95, src/VectorFunction.cpp:53-61) */
-  if (gradp) {
-    gdiff.t0 = time_1;
-    gdiff.t1 = t_n;
-  }
+  if (gradp) orc_nhn_stage_function(g, gradp, t_n, 1, &gdiff);
   orc_solve_pressure(g, p_buf, buf, gradp ? &gdiff : NULL, dt_1);
   add_pressure(g, p, p_buf);
   project(g, buf, p_buf, dt_1);
@@ -880,10 +899,7 @@ void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3],
     }
   }
   orc_apply_bc(g, buf2[0], buf2[1], buf2[2], ev, time_2);
-  if (gradp) {
-    gdiff.t0 = time_2;
-    gdiff.t1 = time_1;
-  }
+  if (gradp) orc_nhn_stage_function(g, gradp, t_n, 2, &gdiff);
   orc_solve_pressure(g, p_buf, buf2, gradp ? &gdiff : NULL, dt_2);
   add_pressure(g, p, p_buf);
   project(g, buf2, p_buf, dt_2);
@@ -906,10 +922,7 @@ void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3],
     }
   }
   orc_apply_bc(g, vel[0], vel[1], vel[2], ev, final_time);
-  if (gradp) {
-    gdiff.t0 = final_time;
-    gdiff.t1 = time_2;
-  }
+  if (gradp) orc_nhn_stage_function(g, gradp, t_n, 3, &gdiff);
   orc_solve_pressure(g, p_buf, vel, gradp ? &gdiff : NULL, dt_3);
   add_pressure(g, p, p_buf);
   project(g, vel, p_buf, dt_3);

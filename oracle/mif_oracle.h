@@ -77,6 +77,11 @@ void orc_timestep(const orc_grid *g, double *const vel[3], double *const buf[3],
                   const mif_function *grad_p, double t_n, double *p,
                   double *p_buf);
 
+/* The pressure-gradient boundary function that timestep_nhn passes to the
+ * pressure solve of stage 1, 2 or 3 (src/Timestep.cpp:89-95,110,122,134). */
+void orc_nhn_stage_function(const orc_grid *g, const mif_function *gradp, double t_n, int stage,
+                            mif_function *out);
+
 /* Velocity-only RK3 with forcing (src/TimestepVelocity.cpp:60-90).  On return
  * vel holds the new velocity (the reference swaps buffers; the oracle copies). */
 void orc_timestep_velocity(const orc_grid *g, double *const vel[3],
@@ -89,6 +94,13 @@ void orc_timestep_velocity(const orc_grid *g, double *const vel[3],
  * ghosts).  grad_p NULL selects the homogeneous form. */
 void orc_solve_pressure(const orc_grid *g, double *p, double *const vel[3],
                         const mif_function *grad_p, double dt);
+
+/* The right-hand side alone: div(vel)/dt on the owner points of p
+ * (compute_rhs_homogeneous_periodic, src/PressureEquation.cpp:59-61), plus
+ * the Neumann data of grad_p when it is not NULL
+ * (apply_non_homogeneous_neumann, :10-57); other points are left untouched. */
+void orc_pressure_rhs(const orc_grid *g, double *p, double *const vel[3], const mif_function *grad_p,
+                      double dt);
 
 /* The spectral solve alone on a packed owner-point array (i fastest), in
  * place (src/PressureEquation.cpp:65-264). */

@@ -120,6 +120,9 @@ def lib():
                      "orc_redft00_direct"):
             getattr(L, name).argtypes = [ctypes.c_long, dp, dp]
         L.orc_timestep_adjoint.argtypes = [G, dpp, dpp, dpp, dpp, dp, dp]
+        L.orc_extents.argtypes = [G, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
+        L.orc_nhn_stage_function.argtypes = [G, F, ctypes.c_double, ctypes.c_int, F]
+        L.orc_pressure_rhs.argtypes = [G, dp, dpp, F, ctypes.c_double]
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -164,16 +167,11 @@ class Oracle:
 
     # shapes ---------------------------------------------------------------
     def extents(self, s):
-        N = list(self.g.N)
-        p = list(self.g.periodic)
-        e = N[:]
-        if s == 0:
-            e[0] = N[0] if p[0] else N[0] + 1
-        elif s == 1:
-            e[1] = N[1] + 1
-        elif s == 2:
-            e[2] = N[2] + 1
-        return tuple(e)
+        """(sx, sy, sz) of a field of staggering s (0..2 = u, v, w; 3 = pressure),
+        from the oracle's orc_extents."""
+        e = (ctypes.c_long * 3)()
+        lib().orc_extents(ctypes.byref(self.g), s, e)
+        return tuple(int(v) for v in e)
 
     def shape(self, s):
         e = self.extents(s)
@@ -262,6 +260,17 @@ class Oracle:
 
     def momentum_rhs(self, vel, comp, i, j, k):
         return lib().orc_momentum_rhs(ctypes.byref(self.g), _dpp(vel), comp, i, j, k)
+
+    def nhn_stage_function(self, grad_p, t_n, stage) -> MifFunction:
+        """The boundary function timestep_nhn passes to stage 1..3's solve."""
+        out = MifFunction()
+        lib().orc_nhn_stage_function(ctypes.byref(self.g), ctypes.byref(grad_p), t_n, stage, ctypes.byref(out))
+        return out
+
+    def pressure_rhs(self, p, vel, dt, grad_p=None):
+        """div(vel)/dt (plus Neumann data) on the owner points of p, in place."""
+        lib().orc_pressure_rhs(ctypes.byref(self.g), _dp(p), _dpp(vel),
+                               ctypes.byref(grad_p) if grad_p is not None else None, dt)
 
 
 def transform(name: str, x: np.ndarray) -> np.ndarray:
