@@ -215,6 +215,15 @@ int mif_field_dot(const mif_field *a, const mif_field *b, double *out);
  * (writeVTK/writeDat, src/VTKDatExport.cpp) fetch only the points they
  * print through this.  Allocates device scratch per call; synchronises. */
 int mif_field_gather(const mif_field *f, const int32_t *ijk, int64_t n, double *out);
+/* The same gather with no device allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving or output loops, harnesses that forbid hipMalloc).
+ * The indices and values pass through a fixed device array of the library
+ * (one per GPU, held until the process ends) instead of per-call buffers, so
+ * n is limited to 1048576 points; larger n returns MIF_ERR_INVALID_ARGUMENT.
+ * All calls on one GPU share that array: calls that overlap in time on one
+ * GPU (contexts used from several threads) race.  Synchronises. */
+int mif_field_gather_noalloc(const mif_field *f, const int32_t *ijk, int64_t n, double *out);
 
 /* ---- measurement ------------------------------------------------------------
  * Kernel timing with HIP events recorded on the context's stream around

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_set>
@@ -21,9 +22,42 @@
 
 using namespace mif;
 
+// Device scratch of mif_field_gather_noalloc: the point indices (three int32
+// per point) at offset 0, the gathered FP64 values at kGatherNoallocValues,
+// for up to kGatherNoallocMaxPoints points.  Every GPU the library runs on
+// holds its own copy until the process ends.  Host code reaches a device's
+// copy only through the address hipGetSymbolAddress returns with that device
+// current (kept per device in g_scratch_addr and in each context); the
+// array's name is a host-side stand-in that must never be passed to a kernel
+// or a copy.
+constexpr int64_t kGatherNoallocMaxPoints = int64_t(1) << 20;
+constexpr size_t kGatherNoallocValues = size_t(12) << 20;  // 12 bytes per point, 256-byte aligned
+constexpr size_t kScratchBytes = kGatherNoallocValues + (size_t(8) << 20);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current; looked up
+// once per device.
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = ptr;
+  }
+  *out = g_scratch_addr[device];
+  return hipSuccess;
+}
 
 struct TimerRec {
   int cls;
@@ -78,6 +112,10 @@ struct mif_ctx {
   Red3* red_out = nullptr;
   int nred = 1024;
   std::string err;
+  // this device's copy of the no-allocation scratch array (nullptr if its
+  // address could not be obtained; scratch_err then says why)
+  void* scratch = nullptr;
+  std::string scratch_err;
   // kernel timing
   bool timing = false;
   std::vector<TimerRec> pending;
@@ -851,6 +889,9 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
   HIPCHK(nullptr, hipMalloc(&c->partial, c->nred * sizeof(Red3)));
   HIPCHK(nullptr, hipMalloc(&c->red_out, sizeof(Red3)));
   HIPCHK(nullptr, hipStreamSynchronize(c->stream));
+  // the no-allocation entry point's scratch on this device
+  const hipError_t se = scratch_address(device, &c->scratch);
+  if (se != hipSuccess) c->scratch_err = hipGetErrorString(se);
   return MIF_OK;
 }
 
@@ -1473,6 +1514,37 @@ int mif_field_gather(const mif_field* f, const int32_t* ijk, int64_t n, double* 
   (void)hipFree(dijk);
   (void)hipFree(dout);
   if (err != hipSuccess) return fail(c, MIF_ERR_HIP, "gather: %s", hipGetErrorString(err));
+  return MIF_OK;
+}
+
+int mif_field_gather_noalloc(const mif_field* f, const int32_t* ijk, int64_t n, double* out) {
+  if (!f || !f->ctx || n < 0 || (n > 0 && (!ijk || !out)))
+    return fail(f ? f->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "bad gather arguments");
+  mif_ctx* c = f->ctx;
+  if (n > kGatherNoallocMaxPoints)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "gather_noalloc: %lld points requested, at most %lld", (long long)n,
+                (long long)kGatherNoallocMaxPoints);
+  if (n == 0) return MIF_OK;
+  if (!c->scratch)
+    return fail(c, MIF_ERR_HIP, "gather_noalloc: no device scratch on device %d (%s)", c->device,
+                c->scratch_err.c_str());
+  int e[3];
+  extents(c->g, f->stag, e);
+  for (int64_t q = 0; q < n; q++)
+    for (int a = 0; a < 3; a++)
+      if (ijk[3 * q + a] < 0 || ijk[3 * q + a] >= e[a])
+        return fail(c, MIF_ERR_INVALID_ARGUMENT, "gather point %lld outside the field", (long long)q);
+  HIPCHK(c, hipSetDevice(c->device));
+  int* dijk = (int*)c->scratch;
+  double* dout = (double*)((char*)c->scratch + kGatherNoallocValues);
+  HIPCHK(c, hipMemcpyAsync(dijk, ijk, 3 * n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  if (is_fp64(c))
+    launch_gather<double>(c->L, (const double*)f->data, dijk, (long)n, dout, c->stream);
+  else
+    launch_gather<float>(c->L, (const float*)f->data, dijk, (long)n, dout, c->stream);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MIF_OK;
 }
 

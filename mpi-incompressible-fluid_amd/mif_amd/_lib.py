@@ -116,6 +116,7 @@ SIGNATURES = {
     "mif_field_copy": [_VP, _VP],
     "mif_field_dot": [_VP, _VP, _DP],
     "mif_field_gather": [_VP, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, _DP],
+    "mif_field_gather_noalloc": [_VP, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, _DP],
     "mif_ctx_set_timing": [_VP, _I],
     "mif_ctx_kernel_time": [_VP, _I, _DP, _U64P],
     "mif_kernel_class_name": [_I],
