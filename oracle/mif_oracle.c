@@ -642,6 +642,10 @@ static double eig_periodic(long index, double delta, long N) {
 
 typedef enum { AX_FWD, AX_INV } ax_dir;
 
+/* Per-line transforms by O(N^2) direct sums instead of the FFT: a second
+ * FP64 evaluation order of the same solve (orc_poisson_packed_direct). */
+static int g_direct_transforms = 0;
+
 /* Transform every line of b (extents n[3], x fastest) along axis a. */
 static void transform_axis(const orc_grid *g, double *b, const long n[3], int a, ax_dir dir,
                            double norm) {
@@ -665,7 +669,14 @@ static void transform_axis(const orc_grid *g, double *b, const long n[3], int a,
       for (long q1 = 0; q1 < nl1; q1++) {
         double *base = b + q1 * s1 + q2 * s2;
         for (long t = 0; t < len; t++) lin[t] = base[t * stride];
-        if (per) {
+        if (g_direct_transforms) {
+          if (per)
+            (dir == AX_FWD ? orc_r2hc_direct : orc_hc2r_direct)(len, lin, lout);
+          else if (len < 2)
+            lout[0] = lin[0];
+          else
+            orc_redft00_direct(len, lin, lout);
+        } else if (per) {
           if (dir == AX_FWD)
             r2hc_plan(&P, lin, lout, ca, cb);
           else
@@ -723,6 +734,122 @@ void orc_poisson_packed(const orc_grid *g, double *b) {
   transform_axis(g, b, n, 2, AX_INV, nz);
   transform_axis(g, b, n, 1, AX_INV, ny);
   transform_axis(g, b, n, 0, AX_INV, nx);
+}
+
+void orc_poisson_packed_direct(const orc_grid *g, double *b) {
+  g_direct_transforms = 1;
+  orc_poisson_packed(g, b);
+  g_direct_transforms = 0;
+}
+
+void orc_set_direct_transforms(int on) { g_direct_transforms = on ? 1 : 0; }
+
+/* ------------------------------------------------------------------------ */
+/* Cell-centred Neumann axes (no reference counterpart: the reference's      */
+/* Neumann axes are node-centred, FFTW_REDFT00).  N unknowns at the cell      */
+/* centres, ghosts phi_{-1} = phi_0 and phi_N = phi_{N-1}; the eigenvectors   */
+/* of the 1-D operator are cos(pi k (j + 1/2) / N), eigenvalues               */
+/* 2 (cos(pi k / N) - 1) / h^2, so the transform pair is the DCT-II           */
+/*   X_k = sum_j x_j cos(pi k (2j + 1) / (2N))                                */
+/* and the DCT-III y_j = X_0 + 2 sum_{k>=1} X_k cos(pi k (2j + 1) / (2N)),    */
+/* which returns N x.  Direct O(N^2) sums over an exact cosine table.         */
+/* ------------------------------------------------------------------------ */
+static double *cos_table_4n(long N) {
+  double *c = malloc(sizeof(double) * 4 * N);
+  for (long m = 0; m < 4 * N; m++) c[m] = cos(M_PI * (double)m / (2.0 * (double)N));
+  return c;
+}
+
+static void dct2_line(long N, const double *c, const double *x, double *X) {
+  for (long k = 0; k < N; k++) {
+    double s = 0;
+    for (long j = 0; j < N; j++) s += x[j] * c[(k * (2 * j + 1)) % (4 * N)];
+    X[k] = s;
+  }
+}
+
+static void dct3_line(long N, const double *c, const double *X, double *y) {
+  for (long j = 0; j < N; j++) {
+    double s = X[0];
+    for (long k = 1; k < N; k++) s += 2.0 * X[k] * c[(k * (2 * j + 1)) % (4 * N)];
+    y[j] = s;
+  }
+}
+
+void orc_dct2_direct(long n, const double *in, double *out) {
+  double *c = cos_table_4n(n);
+  dct2_line(n, c, in, out);
+  free(c);
+}
+
+void orc_dct3_direct(long n, const double *in, double *out) {
+  double *c = cos_table_4n(n);
+  dct3_line(n, c, in, out);
+  free(c);
+}
+
+static double eig_cell(long index, double delta, long N) {
+  return 2.0 * (cos(M_PI * (double)index / (double)N) - 1.0) / (delta * delta);
+}
+
+/* DCT-II (AX_FWD) or DCT-III divided by N (AX_INV) along axis a. */
+static void transform_axis_cell(double *b, const long n[3], int a, ax_dir dir) {
+  const long len = n[a];
+  const long stride = a == 0 ? 1 : (a == 1 ? n[0] : n[0] * n[1]);
+  const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
+  const long nl1 = n[b1], nl2 = n[b2];
+  const long s1 = b1 == 0 ? 1 : n[0], s2 = b2 == 1 ? n[0] : n[0] * n[1];
+  double *c = cos_table_4n(len);
+#pragma omp parallel
+  {
+    double *lin = malloc(sizeof(double) * len), *lout = malloc(sizeof(double) * len);
+#pragma omp for collapse(2) schedule(static)
+    for (long q2 = 0; q2 < nl2; q2++)
+      for (long q1 = 0; q1 < nl1; q1++) {
+        double *base = b + q1 * s1 + q2 * s2;
+        for (long t = 0; t < len; t++) lin[t] = base[t * stride];
+        if (dir == AX_FWD) {
+          dct2_line(len, c, lin, lout);
+          for (long t = 0; t < len; t++) base[t * stride] = lout[t];
+        } else {
+          dct3_line(len, c, lin, lout);
+          for (long t = 0; t < len; t++) base[t * stride] = lout[t] / (double)len;
+        }
+      }
+    free(lin);
+    free(lout);
+  }
+  free(c);
+}
+
+void orc_poisson_packed_cell(const orc_grid *g, const int cell[3], double *b) {
+  long n[3];
+  for (int a = 0; a < 3; a++) n[a] = cell[a] ? g->Ng[a] - 1 : g->npts[a];
+  for (int a = 0; a < 3; a++) {
+    if (cell[a])
+      transform_axis_cell(b, n, a, AX_FWD);
+    else
+      transform_axis(g, b, n, a, AX_FWD, 1.0);
+  }
+  double *lam[3];
+  for (int a = 0; a < 3; a++) {
+    lam[a] = malloc(sizeof(double) * n[a]);
+    for (long i = 0; i < n[a]; i++)
+      lam[a][i] = cell[a] ? eig_cell(i, g->h[a], n[a])
+                          : (g->periodic[a] ? eig_periodic(i, g->h[a], n[a]) : eig_neumann(i, g->h[a], n[a]));
+  }
+#pragma omp parallel for schedule(static)
+  for (long k = 0; k < n[2]; k++)
+    for (long j = 0; j < n[1]; j++)
+      for (long i = 0; i < n[0]; i++) b[i + n[0] * (j + n[1] * k)] *= 1 / (lam[0][i] + lam[1][j] + lam[2][k]);
+  b[0] = 0;
+  for (int a = 0; a < 3; a++) free(lam[a]);
+  for (int a = 2; a >= 0; a--) {
+    if (cell[a])
+      transform_axis_cell(b, n, a, AX_INV);
+    else
+      transform_axis(g, b, n, a, AX_INV, (double)(g->Ng[a] - 1) * (g->periodic[a] ? 1.0 : 2.0));
+  }
 }
 
 /* Owner-point ranges of the pressure tensor (STAGGERED_TENSOR_ITERATE_OVER_

@@ -106,6 +106,27 @@ void orc_pressure_rhs(const orc_grid *g, double *p, double *const vel[3], const 
  * place (src/PressureEquation.cpp:65-264). */
 void orc_poisson_packed(const orc_grid *g, double *b);
 
+/* The spectral solve with cell-centred Neumann axes (no reference
+ * counterpart): on an axis with cell[a] != 0 (a wall axis) the unknowns are
+ * the N_g - 1 cell centres, the ghosts mirror the first/last cell and the
+ * transform pair is the DCT-II / DCT-III.  b is the packed array (extents
+ * N_g - 1 on cell-centred axes, npts elsewhere), solved in place with a zero
+ * constant mode. */
+void orc_poisson_packed_cell(const orc_grid *g, const int cell[3], double *b);
+/* X_k = sum_j x_j cos(pi k (2j + 1) / (2n)) and its inverse up to the factor
+ * n, y_j = X_0 + 2 sum_{k>=1} X_k cos(pi k (2j + 1) / (2n)); direct sums. */
+void orc_dct2_direct(long n, const double *in, double *out);
+void orc_dct3_direct(long n, const double *in, double *out);
+
+/* The same solve with every 1-D transform done by O(N^2) direct sums: the
+ * same mathematics in another FP64 operation order, to measure how much two
+ * correct FP64 solves differ (not thread-safe against concurrent calls of
+ * orc_poisson_packed). */
+void orc_poisson_packed_direct(const orc_grid *g, double *b);
+/* Every later solve (orc_solve_pressure, orc_timestep, ...) with direct-sum
+ * transforms (on = 1) or the FFTs (on = 0, the default). */
+void orc_set_direct_transforms(int on);
+
 /* adjust_pressure (src/PressureEquation.cpp:288-343). */
 void orc_adjust_pressure(const orc_grid *g, double *p, const mif_function *fn,
                          double t);

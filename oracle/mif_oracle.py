@@ -117,12 +117,15 @@ def lib():
         L.orc_momentum_rhs.argtypes = [G, dpp, ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.c_long]
         L.orc_momentum_rhs.restype = ctypes.c_double
         for name in ("orc_r2hc", "orc_hc2r", "orc_redft00", "orc_r2hc_direct", "orc_hc2r_direct",
-                     "orc_redft00_direct"):
+                     "orc_redft00_direct", "orc_dct2_direct", "orc_dct3_direct"):
             getattr(L, name).argtypes = [ctypes.c_long, dp, dp]
         L.orc_timestep_adjoint.argtypes = [G, dpp, dpp, dpp, dpp, dp, dp]
         L.orc_extents.argtypes = [G, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
         L.orc_nhn_stage_function.argtypes = [G, F, ctypes.c_double, ctypes.c_int, F]
         L.orc_pressure_rhs.argtypes = [G, dp, dpp, F, ctypes.c_double]
+        L.orc_poisson_packed_cell.argtypes = [G, ctypes.POINTER(ctypes.c_int), dp]
+        L.orc_poisson_packed_direct.argtypes = [G, dp]
+        L.orc_set_direct_transforms.argtypes = [ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -240,6 +243,20 @@ class Oracle:
         assert b.shape == tuple(reversed(self.npts))
         lib().orc_poisson_packed(ctypes.byref(self.g), _dp(b))
 
+    def poisson_packed_direct(self, b: np.ndarray):
+        """The same solve with O(N^2) direct-sum transforms: another FP64
+        evaluation order of the same mathematics."""
+        assert b.shape == tuple(reversed(self.npts))
+        lib().orc_poisson_packed_direct(ctypes.byref(self.g), _dp(b))
+
+    def poisson_packed_cell(self, b: np.ndarray, cell):
+        """In-place spectral solve with cell-centred Neumann axes (cell[a] != 0):
+        b is (nz, ny, nx) with N_g - 1 points on those axes."""
+        c = (ctypes.c_int * 3)(*[1 if x else 0 for x in cell])
+        n = [self.g.Ng[a] - 1 if cell[a] else self.g.npts[a] for a in range(3)]
+        assert b.shape == (n[2], n[1], n[0])
+        lib().orc_poisson_packed_cell(ctypes.byref(self.g), c, _dp(b))
+
     def adjust_pressure(self, p, fn, t):
         lib().orc_adjust_pressure(ctypes.byref(self.g), _dp(p), ctypes.byref(fn), t)
 
@@ -279,6 +296,12 @@ def transform(name: str, x: np.ndarray) -> np.ndarray:
     out = np.zeros_like(x)
     getattr(lib(), "orc_" + name)(len(x), _dp(x), _dp(out))
     return out
+
+
+def set_direct_transforms(on: bool):
+    """Run every later solve with O(N^2) direct-sum transforms (True) or the
+    FFTs (False, the default)."""
+    lib().orc_set_direct_transforms(1 if on else 0)
 
 
 def num_threads() -> int:

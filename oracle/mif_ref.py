@@ -69,6 +69,21 @@ def _evaluator():
     return ctypes.cast(mo.lib().orc_fn_eval, ctypes.c_void_p)
 
 
+EVAL_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                           ctypes.c_double, ctypes.c_double)
+
+
+class PyFunction:
+    """A vector function given as three Python callables f(t, x, y, z),
+    passed to the reference code in place of a descriptor: `fn` for the
+    context argument and `evaluator` for the function pointer."""
+
+    def __init__(self, fns):
+        self.fns = fns
+        self._cb = EVAL_FN(lambda _ctx, c, t, x, y, z: float(self.fns[c](t, x, y, z)))
+        self.evaluator = ctypes.cast(self._cb, ctypes.c_void_p)
+
+
 class Reference:
     """The reference's own code at one rank on one grid descriptor."""
 
@@ -99,8 +114,12 @@ class Reference:
                       comp, t, 1 if include_border else 0)
 
     def apply_bc(self, vel, fn, t: float):
-        lib().ref_apply_bc(ctypes.byref(self.grid), *[_dp(self._check(vel[c], c)) for c in range(3)],
-                           _evaluator(), ctypes.addressof(fn), t)
+        """fn: an oracle descriptor (mif_function) or a PyFunction."""
+        if isinstance(fn, PyFunction):
+            ev, ctx = fn.evaluator, None
+        else:
+            ev, ctx = _evaluator(), ctypes.addressof(fn)
+        lib().ref_apply_bc(ctypes.byref(self.grid), *[_dp(self._check(vel[c], c)) for c in range(3)], ev, ctx, t)
 
     def apply_periodic(self, f: np.ndarray, s: int):
         lib().ref_apply_periodic(ctypes.byref(self.grid), s, _dp(self._check(f, s)))
