@@ -241,6 +241,68 @@ def adjoint_leg(n: int, steps: int, k: int, directions: int, eps: float = 1e-6):
 ADJ_FP32_SWEEPS = 0
 
 
+def poisson_leg(n: int, steps: int, warmup: int):
+    """The pressure Poisson solve alone at n^3, cell-centred: periodic x and z
+    (n points), n cells in y between walls (the DCT-II axis), domain
+    [0,2pi]x[-1,1]x[0,pi]; right-hand side N(0,1) from numpy's PCG64 with
+    seed 20261202, projected to zero mean.  Per form (FP64; FP32 transforms
+    with 0, 1 and 2 FP64 defect sweeps): the median solve time over
+    synchronised blocks, the residual of the exact 7-point operator and the
+    distance from the FP64 solution."""
+    import numpy as np
+
+    import mif_amd as mif
+
+    gs = mif.GridSpec(n + 1, n + 1, n + 1, 2 * np.pi, 2.0, np.pi, 0.0, -1.0, 0.0, 1.0, 1.0, 1, (True, False, True))
+    cell = (False, True, False)
+    b = np.random.default_rng(20261202).standard_normal((n, n, n))
+    b -= b.mean()
+    h = (2 * np.pi / n, 2.0 / n, np.pi / n)
+
+    def residual(phi):
+        r = b.copy()
+        for a in range(3):
+            ax = 2 - a
+            if a != 1:
+                lo, hi = np.roll(phi, 1, axis=ax), np.roll(phi, -1, axis=ax)
+            else:
+                idx = np.arange(n)
+                lo = np.take(phi, np.maximum(idx - 1, 0), axis=ax)
+                hi = np.take(phi, np.minimum(idx + 1, n - 1), axis=ax)
+            r -= (lo - 2 * phi + hi) / (h[a] * h[a])
+        return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+    out = {"workload": f"poisson_cell_centred_{n}^3", "unknowns": n ** 3, "periodic": "x,z",
+           "cell_centred_neumann": "y (DCT-II)", "rhs": "N(0,1), PCG64 seed 20261202, zero mean"}
+    sols = {}
+    for name, prec, sweeps in (("fp64", mif.FP64, 0), ("mixed_s0", mif.MIXED, 0), ("mixed_s1", mif.MIXED, 1),
+                               ("mixed_s2", mif.MIXED, 2)):
+        ctx = mif.Context(gs, precision=prec, device=0, cell_centred=cell)
+        if prec == mif.MIXED:
+            ctx.set_defect_sweeps(sweeps)
+        ctx.poisson_set_rhs(b)
+        for _ in range(warmup):
+            ctx.poisson_run()
+        blocks = []
+        for nb in block_sizes(steps):
+            ctx.synchronize()
+            b0 = time.perf_counter()
+            for _ in range(nb):
+                ctx.poisson_run()
+            ctx.synchronize()
+            blocks.append((time.perf_counter() - b0) / nb)
+        sols[name] = ctx.poisson_get_solution()
+        ctx.close()
+        step_s = sorted(blocks)[len(blocks) // 2]
+        out[name] = {"ms_per_solve": step_s * 1e3, "unknowns_per_s": n ** 3 / step_s,
+                     "residual_rel_l2": residual(sols[name])}
+        if name != "fp64":
+            out[name]["error_rel_l2_vs_fp64"] = float(np.linalg.norm(sols[name] - sols["fp64"]) /
+                                                      np.linalg.norm(sols["fp64"]))
+            out[name]["speed_vs_fp64"] = out["fp64"]["ms_per_solve"] / out[name]["ms_per_solve"]
+    return out
+
+
 def velocity_leg(n: int, steps: int, warmup: int):
     """The reference's own benchmark (test/benchmark.cpp:26-61):
     timestep_velocity on the unit cube with n nodes per axis, all walls,
@@ -459,6 +521,8 @@ def main():
     ap.add_argument("--adjoint-n", type=int, default=128, help="intervals per axis of the gradient workload (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=32)
     ap.add_argument("--adjoint-directions", type=int, default=16)
+    ap.add_argument("--poisson-n", type=int, default=256,
+                    help="size of the stand-alone cell-centred Poisson leg (0 = skip)")
     ap.add_argument("--velocity-n", type=int, default=256,
                     help="nodes per axis of the timestep_velocity leg (the reference's own benchmark; 0 = skip)")
     ap.add_argument("--large", type=int, default=0, metavar="STEPS",
@@ -543,6 +607,10 @@ def main():
     adj = (adjoint_leg(args.adjoint_n, args.adjoint_steps, 8, args.adjoint_directions)
            if args.adjoint_n > 0 else None)
     large = large_leg(args.large) if args.large > 0 else None
+    poisson = None
+    if args.poisson_n > 0:
+        print(f"[bench] Poisson solve alone: {args.poisson_n}^3 cell-centred", file=sys.stderr, flush=True)
+        poisson = poisson_leg(args.poisson_n, args.steps, args.warmup)
     vstep = None
     if args.velocity_n > 0:
         print(f"[bench] timestep_velocity: {args.velocity_n}^3 nodes", file=sys.stderr, flush=True)
@@ -575,6 +643,7 @@ def main():
         "kernels": table,
         "adjoint": adj,
         "velocity_step": vstep,
+        "poisson": poisson,
     }
     if large:
         line["large"] = large

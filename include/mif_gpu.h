@@ -46,6 +46,21 @@ typedef enum mif_precision {
  * tables and scratch live in the context.  precision is a mif_precision;
  * device is the HIP device ordinal (-1 = current). */
 int mif_ctx_create(const mif_grid *grid, int precision, int device, mif_ctx **out);
+/* A context for stand-alone Poisson solves with cell-centred Neumann axes
+ * (no counterpart in the reference, whose Neumann axes are node-centred:
+ * FFTW_REDFT00, src/PressureSolverStructures.cpp:23-28).  On an axis with
+ * cell_centred[a] != 0 (walls, not periodic) the unknowns are the N_g - 1
+ * cell centres between the N_g nodes of the grid, the walls reflect the
+ * first/last cell (phi_{-1} = phi_0), the operator's rows are (-1, 1),
+ * (1, -2, 1), (1, -1) over h^2, and the transform pair is the DCT-II /
+ * DCT-III with eigenvalues 2 (cos(pi k / N) - 1) / h^2.  The y axis only,
+ * with 64, 128, 256, 512 or 1024 cells (else MIF_ERR_UNSUPPORTED).  Such a
+ * context has no staggered fields (mif_field_create returns
+ * MIF_ERR_UNSUPPORTED); it serves mif_poisson_set_rhs / mif_poisson_run /
+ * mif_poisson_get_solution, whose packed arrays then hold N_g - 1 points
+ * along that axis. */
+int mif_ctx_create_cell_centred(const mif_grid *grid, const int cell_centred[3], int precision, int device,
+                                mif_ctx **out);
 /* Frees everything the context owns (PressureSolverStructures::
  * ~PressureSolverStructures, src/PressureSolverStructures.cpp:72-97).  Fields
  * must be destroyed first. */
@@ -113,6 +128,30 @@ int mif_field_apply_periodic_bc(mif_field *f);
  * 47-51).  vel = {u, v, w}. */
 int mif_velocity_apply_bc(mif_field *const vel[3], const mif_function *exact_velocity, double t);
 
+/* ---- boundary data from host callbacks ---------------------------------------
+ * The reference evaluates its boundary data through std::function callbacks
+ * (include/VectorFunction.h:15-53) at every stage.  The entry points above
+ * take closed-form descriptors the device evaluates; a callback without one
+ * is evaluated on the host into face tables that these variants take:
+ *   velocity face table: for u, v, w in turn, the faces z-, z+, y-, y+, x-, x+
+ *     of the non-periodic axes, each a plane over the component's extents
+ *     along the two other axes, lower axis fastest (z faces i + ex*j, y faces
+ *     i + ex*k, x faces j + ey*k), holding the value that
+ *     VelocityTensor::apply_bc writes at that point of that face
+ *     (src/VelocityTensor.cpp:36-233): the exact value for a tangential
+ *     component, the extrapolated value for the normal one;
+ *   Neumann table: the six faces of the node extents in the same order and
+ *     indexing, holding the normal component of the pressure-gradient data at
+ *     each wall node (x component on x faces, ...), as
+ *     apply_non_homogeneous_neumann reads it (src/PressureEquation.cpp:10-57).
+ * mif_bc_table_size gives the element count of one table of each kind.  The
+ * tables are copied before the call returns.  Edge points of a face that a
+ * later face overwrites, and faces of periodic axes, follow the reference's
+ * order as in the descriptor forms. */
+int mif_bc_table_size(const mif_ctx *ctx, int64_t *velocity_faces, int64_t *neumann_faces);
+/* VelocityTensor::apply_bc with a velocity face table. */
+int mif_velocity_apply_bc_tabulated(mif_field *const vel[3], const double *faces);
+
 /* ---- timesteps --------------------------------------------------------------
  * timestep (include/Timestep.h:16-19, src/Timestep.cpp:151-156): one RK3 step
  * with an incremental projection per stage, homogeneous Neumann/periodic
@@ -131,6 +170,21 @@ int mif_timestep_nhn(mif_field *const vel[3], mif_field *const buf[3], mif_field
  * vel and buf have swapped storage as in the reference. */
 int mif_timestep_velocity(mif_field *const vel[3], mif_field *const buf[3], mif_field *const rhs[3],
                           const mif_function *exact_velocity, const mif_function *forcing, double t_n);
+/* The three timesteps with boundary data from host callbacks: `faces` holds
+ * three velocity face tables back to back, the exact velocity at the stage
+ * times (timestep: t_n + 64/120 dt, then + 16/120 dt, then t_n + dt,
+ * src/Timestep.cpp:99-104; timestep_velocity: t_n + 8/15 dt, t_n + 2/3 dt,
+ * t_n + dt, src/TimestepVelocity.cpp:60-90); `neumann` holds three Neumann
+ * tables of the data each stage's solve adds, exact_pressure_gradient.
+ * get_difference_over_time(new_time, prev_time) = grad p(prev_time) -
+ * grad p(new_time) (src/Timestep.cpp:89-95, src/VectorFunction.cpp:53-61). */
+int mif_timestep_tabulated(mif_field *const vel[3], mif_field *const buf[3], mif_field *const buf2[3],
+                           const double *faces, double t_n, mif_field *pressure, mif_field *pressure_buffer);
+int mif_timestep_nhn_tabulated(mif_field *const vel[3], mif_field *const buf[3], mif_field *const buf2[3],
+                               const double *faces, const double *neumann, double t_n, mif_field *pressure,
+                               mif_field *pressure_buffer);
+int mif_timestep_velocity_tabulated(mif_field *const vel[3], mif_field *const buf[3], mif_field *const rhs[3],
+                                    const double *faces, const mif_function *forcing, double t_n);
 
 /* ---- pressure -------------------------------------------------------------
  * solve_pressure_equation_homogeneous_periodic (include/PressureEquation.h:
@@ -143,6 +197,9 @@ int mif_solve_pressure_homogeneous_periodic(mif_field *pressure, mif_field *cons
 int mif_solve_pressure_non_homogeneous_neumann(mif_field *pressure, mif_field *const vel[3],
                                                const mif_function *exact_pressure_gradient, double t,
                                                double dt);
+/* The same with the Neumann data as one host table (see above). */
+int mif_solve_pressure_non_homogeneous_neumann_tabulated(mif_field *pressure, mif_field *const vel[3],
+                                                         const double *neumann, double dt);
 /* The spectral solve alone (src/PressureEquation.cpp:65-264 between
  * copy_from_staggered and copy_to_staggered): the right-hand side is read
  * from the owner points of `pressure`, the solution replaces it (owner

@@ -88,10 +88,20 @@ struct mif_ctx {
   FftPlan plan[3]{};
   double* W64[3]{};
   float* W32[3]{};
+  double* H64[3]{};  // half-angle tables of cell-centred axes
+  float* H32[3]{};
   double* lam[3]{};
   double inv_norm = 1;
   long bp0 = 0, bplane = 0, belems = 0;
   void* B = nullptr;       // packed working buffer (field precision)
+  void* B2 = nullptr;      // mixed form: second FP32 packed buffer (the correction of the one-sweep path)
+  Red3* xpart = nullptr;   // mixed form: per-workgroup sums of the fused x passes
+  int nxpart = 0;
+  Red3* red_b = nullptr;   // mixed form: (W.b, W.1) of the right-hand side
+  Red3* red_phi = nullptr; // mixed form: (W.phi, W.1) of the stored increment
+  // the increment the last solve stored still holds its weighted mean
+  // (red_phi); the next projection subtracts it from the pressure update
+  const Red3* phi_mean = nullptr;
   double* B64 = nullptr;   // mixed form: FP64 right-hand side
   double* P64 = nullptr;   // mixed form: FP64 accumulated solution
   double* R64 = nullptr;   // stand-alone Poisson runs: stored FP64 right-hand side
@@ -101,6 +111,7 @@ struct mif_ctx {
   void* adj[4][3]{};
   double* adj_partial = nullptr;
   double* adj_fbar = nullptr;
+  bool adj_ready = false;  // every adjoint buffer allocated
   // velocity-only step: sin/cos of the coordinates of every axis,
   // unshifted and staggered (launch_trig_tables), filled on first use
   double* trig = nullptr;
@@ -116,6 +127,10 @@ struct mif_ctx {
   // address could not be obtained; scratch_err then says why)
   void* scratch = nullptr;
   std::string scratch_err;
+  // host-evaluated boundary data: three velocity face tables, then three
+  // Neumann tables (allocated on first use)
+  double* ftab = nullptr;
+  long ftab_elems = 0;
   // kernel timing
   bool timing = false;
   std::vector<TimerRec> pending;
@@ -238,6 +253,11 @@ double eig_neumann(size_t index, double delta, size_t N) {
 }
 double eig_periodic(size_t index, double delta, size_t N) {
   return 2.0 * (std::cos(2.0 * M_PI * index / N) - 1.0) / (delta * delta);
+}
+// Cell-centred Neumann axis of N cells (no reference counterpart): the
+// eigenvalue of cos(pi k (j + 1/2) / N).
+double eig_cell_neumann(size_t index, double delta, size_t N) {
+  return 2.0 * (std::cos(M_PI * index / N) - 1.0) / (delta * delta);
 }
 
 template <typename T>
@@ -494,12 +514,87 @@ int pressure_solve_mixed64(mif_ctx* c, mif_field* p, mif_field* const vel[3], do
   return MIF_OK;
 }
 
+// Mixed form with one FP64 defect sweep and FP32 transforms, fused into the
+// transform passes (homogeneous data, an x length with a fused kernel):
+//   x forward + divergence: b = div(u)/dt in FP64, stored (b64) with its
+//     weighted sums (W.b, W.1), and its x transform in FP32;
+//   y, z (scale), y, x inverse: phi_0 in FP32 (B);
+//   residual: r = (b - mean_b) - L phi_0 in FP64 (FP32 phi_0 read exactly),
+//     stored in FP32 (B2);
+//   x, y, z, y transforms of r; x inverse + unpack: phi = phi_0 + delta in
+//     FP64, stored as the FP32 increment with its periodic images, and its
+//     weighted sums (W.phi, W.1).
+// The increment's weighted mean (FP32 rounding of the zeroed constant mode)
+// is left in the field: the projection subtracts it from the pressure update
+// (c->phi_mean), stand-alone solves subtract it from the field.
+int pressure_solve_mixed_dc1(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt) {
+  float* B32 = (float*)c->B;
+  float* R32 = (float*)c->B2;
+  const SpectralScale sc = scale_of<float>(c);
+  FusedIO io{vel[0]->data, vel[1]->data, vel[2]->data, p->data, c->L, dt};
+  io.dc = 1;
+  io.b64 = c->B64;
+  io.partial = c->xpart;
+  {
+    KTimer t(c, MIF_KC_FFT_X_DIV);
+    launch_fft_pow2<float>(c->g, 0, 0, c->plan[0], sc, B32, c->bp0, c->bplane, &io, c->stream);
+  }
+  {
+    KTimer t(c, MIF_KC_DEFECT);
+    launch_reduce_partials(c->xpart, c->nxpart, c->red_b, c->stream);
+  }
+  spectral_middle<float>(c, B32);
+  {
+    KTimer t(c, MIF_KC_FFT_X);
+    launch_fft_axis<float>(c->g, 0, +1, 0, c->plan[0], sc, B32, c->bp0, c->bplane, c->stream);
+  }
+  {
+    KTimer t(c, MIF_KC_DEFECT);
+    launch_residual32(c->g, c->B64, B32, c->red_b, R32, c->bp0, c->bplane, c->stream);
+  }
+  {
+    KTimer t(c, MIF_KC_FFT_X);
+    launch_fft_axis<float>(c->g, 0, -1, 0, c->plan[0], sc, R32, c->bp0, c->bplane, c->stream);
+  }
+  spectral_middle<float>(c, R32);
+  io.add32 = B32;
+  {
+    KTimer t(c, MIF_KC_FFT_X_UNPACK);
+    launch_fft_pow2<float>(c->g, 0, 1, c->plan[0], sc, R32, c->bp0, c->bplane, &io, c->stream);
+  }
+  {
+    KTimer t(c, MIF_KC_DEFECT);
+    launch_reduce_partials(c->xpart, c->nxpart, c->red_phi, c->stream);
+  }
+  LAUNCHCHK(c);
+  c->phi_mean = c->red_phi;
+  return MIF_OK;
+}
+
+bool dc1_fused(const mif_ctx* c, int nhn) {
+  return !is_fp64(c) && !mixed_solve_fp64(c) && c->sweeps == 1 && !nhn && c->nxpart > 0 &&
+         fft_fast_len(c->plan[0].L);
+}
+
 int pressure_solve_any(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
                        int nhn) {
+  c->phi_mean = nullptr;
   if (is_fp64(c)) return pressure_solve<double>(c, p, vel, dt, gp, tg, nhn);
   if (mixed_solve_fp64(c)) return pressure_solve_mixed64(c, p, vel, dt, gp, tg, nhn);
+  if (dc1_fused(c, nhn)) return pressure_solve_mixed_dc1(c, p, vel, dt);
   if (c->sweeps > 0 || nhn) return pressure_solve_mixed(c, p, vel, dt, gp, tg, nhn);
   return pressure_solve<float>(c, p, vel, dt, gp, tg, nhn);
+}
+
+// A stand-alone solve hands the increment to the caller: its weighted mean,
+// if the solve left one, is removed from the field here.
+int finish_standalone_solve(mif_ctx* c, mif_field* p) {
+  if (c->phi_mean) {
+    launch_sub_mean_field(c->g, c->L, fptr<float>(p), c->phi_mean, c->stream);
+    c->phi_mean = nullptr;
+    LAUNCHCHK(c);
+  }
+  return MIF_OK;
 }
 
 int pressure_solve_any(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
@@ -537,61 +632,66 @@ template <typename T>
 int project(mif_ctx* c, mif_field* const v[3], mif_field* p, mif_field* phi, double dts) {
   KTimer tm(c, MIF_KC_PROJECT);
   launch_project<T>(c->g, c->L, fptr<T>(v[0]), fptr<T>(v[1]), fptr<T>(v[2]), fptr<T>(p), fptr<T>(phi), dts,
-                    c->stream);
+                    c->stream, c->phi_mean);
+  c->phi_mean = nullptr;
   return MIF_OK;
 }
 
-// FULL_TIMESTEP (src/Timestep.cpp:97-141).
+// Stage times of FULL_TIMESTEP (src/Timestep.cpp:99-104): the boundary data
+// of stage s are taken at stage_time[s] and its Neumann data are
+// f(prev_time[s]) - f(stage_time[s]).
+struct StageTimes {
+  double dt_s[4];
+  double time[4];  // [0] = t_n
+};
+
+StageTimes stage_times(double dt, double t_n) {
+  StageTimes st{};
+  st.dt_s[1] = 64.0 / 120.0 * dt;
+  st.dt_s[2] = 16.0 / 120.0 * dt;
+  st.dt_s[3] = 40.0 / 120.0 * dt;
+  st.time[0] = t_n;
+  st.time[1] = t_n + st.dt_s[1];
+  st.time[2] = st.time[1] + st.dt_s[2];
+  st.time[3] = t_n + dt;
+  return st;
+}
+
+// FULL_TIMESTEP (src/Timestep.cpp:97-141).  ev[s - 1] is the boundary data
+// of stage s (a descriptor evaluated at the stage time, or a host table);
+// gd, when not null, the Neumann data of the three stages' solves.
 template <typename T>
 int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
-                  const FnDesc& ev, const mif_function* gradp, double t_n, mif_field* p, mif_field* pb) {
+                  const FnDesc ev[3], const FnDesc* gd, double t_n, mif_field* p, mif_field* pb) {
   const double dt = c->g.dt;
-  const double dt_1 = 64.0 / 120.0 * dt;
-  const double time_1 = t_n + dt_1;
-  const double dt_2 = 16.0 / 120.0 * dt;
-  const double time_2 = time_1 + dt_2;
-  const double dt_3 = 40.0 / 120.0 * dt;
-  const double final_time = t_n + dt;
-  const int nhn = gradp != nullptr;
-  // TimeVectorFunction::get_difference_over_time(new_time, prev_time) yields
-  // f(prev_time) - f(new_time) (src/Timestep.cpp:89-95, src/VectorFunction.cpp:53-61)
-  FnDesc gd{};
-  if (nhn) {
-    gd = make_fn(gradp);
-    gd.base = gradp->kind;
-    gd.kind = MIF_FN_TIME_DIFFERENCE;
-  }
+  const StageTimes st = stage_times(dt, t_n);
+  const int nhn = gd != nullptr;
+  const FnDesc none{};
   int rc;
   // Stage 1 (src/Timestep.cpp:107-117)
   {
     const double a1 = 64.0 / 120.0 * dt;
     if ((rc = stage<T>(c, 1, vel, nullptr, buf, buf2, p, a1, 0.0, a1))) return rc;
   }
-  apply_bc<T>(c, buf, ev, time_1);
-  gd.t0 = time_1;
-  gd.t1 = t_n;
-  if ((rc = pressure_solve_any(c, pb, buf, dt_1, gd, 0.0, nhn))) return rc;
-  project<T>(c, buf, p, pb, dt_1);
+  apply_bc<T>(c, buf, ev[0], st.time[1]);
+  if ((rc = pressure_solve_any(c, pb, buf, st.dt_s[1], nhn ? gd[0] : none, 0.0, nhn))) return rc;
+  project<T>(c, buf, p, pb, st.dt_s[1]);
   // Stage 2 (src/Timestep.cpp:119-129)
   {
     const double a1 = -34.0 / 120.0 * dt, a2 = 50.0 / 120.0 * dt;
     if ((rc = stage<T>(c, 2, buf, buf2, buf2, vel, p, a1, a2, a1 + a2))) return rc;
   }
-  apply_bc<T>(c, buf2, ev, time_2);
-  gd.t0 = time_2;
-  gd.t1 = time_1;
-  if ((rc = pressure_solve_any(c, pb, buf2, dt_2, gd, 0.0, nhn))) return rc;
-  project<T>(c, buf2, p, pb, dt_2);
+  apply_bc<T>(c, buf2, ev[1], st.time[2]);
+  if ((rc = pressure_solve_any(c, pb, buf2, st.dt_s[2], nhn ? gd[1] : none, 0.0, nhn))) return rc;
+  project<T>(c, buf2, p, pb, st.dt_s[2]);
   // Stage 3 (src/Timestep.cpp:131-140)
   {
     const double a2 = -50.0 / 120.0 * dt, a3 = 90.0 / 120.0 * dt;
     if ((rc = stage<T>(c, 3, buf2, vel, vel, nullptr, p, 0.0, a3, a2 + a3))) return rc;
   }
-  apply_bc<T>(c, vel, ev, final_time);
-  gd.t0 = final_time;
-  gd.t1 = time_2;
-  if ((rc = pressure_solve_any(c, pb, vel, dt_3, gd, 0.0, nhn))) return rc;
-  project<T>(c, vel, p, pb, dt_3);
+  apply_bc<T>(c, vel, ev[2], st.time[3]);
+  if ((rc = pressure_solve_any(c, pb, vel, st.dt_s[3], nhn ? gd[2] : none, 0.0, nhn))) return rc;
+  project<T>(c, vel, p, pb, st.dt_s[3]);
   LAUNCHCHK(c);
   return MIF_OK;
 }
@@ -599,7 +699,7 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
 // timestep_velocity (src/TimestepVelocity.cpp:60-90).
 template <typename T>
 int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], mif_field* const rhs[3],
-                      const FnDesc& ev, const FnDesc& fo, double t_n) {
+                      const FnDesc ev[3], const FnDesc& fo, double t_n) {
   const double dt = c->g.dt;
   const double time_1 = t_n + (8.0 / 15.0) * dt;
   const double time_2 = t_n + (2.0 / 3.0) * dt;
@@ -628,21 +728,21 @@ int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[
     KTimer t(c, MIF_KC_STAGE);
     launch_vel_stage<T>(1, c->g, c->L, s, c->stream);
   }
-  apply_bc<T>(c, buf, ev, time_1);
+  apply_bc<T>(c, buf, ev[0], time_1);
   for (int q = 0; q < 3; q++) s.X[q] = buf[q]->data;
   s.t = time_1;
   {
     KTimer t(c, MIF_KC_STAGE);
     launch_vel_stage<T>(2, c->g, c->L, s, c->stream);
   }
-  apply_bc<T>(c, vel, ev, time_2);
+  apply_bc<T>(c, vel, ev[1], time_2);
   for (int q = 0; q < 3; q++) s.X[q] = vel[q]->data;
   s.t = time_2;
   {
     KTimer t(c, MIF_KC_STAGE);
     launch_vel_stage<T>(3, c->g, c->L, s, c->stream);
   }
-  apply_bc<T>(c, buf, ev, final_time);
+  apply_bc<T>(c, buf, ev[2], final_time);
   // velocity.swap_data(velocity_buffer) (src/TimestepVelocity.cpp:89)
   for (int q = 0; q < 3; q++) std::swap(vel[q]->data, buf[q]->data);
   LAUNCHCHK(c);
@@ -659,21 +759,43 @@ int velocity_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[
 //   Nb, Sb/R1b from Y; Xb = Y + J_N(X)^T Nb      (stage update transposed)
 //   pbar += b_s G^T Y;  fbar += sum Nb
 // ---------------------------------------------------------------------------
+// Allocates the adjoint workspace on first use.  Every buffer is allocated
+// into a local first; the context takes them only once all succeeded, so a
+// failed call leaves the context without a workspace (and frees what it got)
+// and the next call starts over.
 int adj_workspace(mif_ctx* c) {
-  if (c->adj_partial) return MIF_OK;
+  if (c->adj_ready) return MIF_OK;
   const size_t bytes = c->L.elems * c->esize;
+  void* adj[4][3] = {};
+  double* partial = nullptr;
+  double* fbar = nullptr;
+  hipError_t e = hipSuccess;
   // z_bar of the current and of a later stage, N_bar, x_bar: zero outside
   // the component interiors is an invariant the kernels rely on
-  for (int q = 0; q < 4; q++)
-    for (int cc = 0; cc < 3; cc++) {
-      HIPCHK(c, alloc_field_storage(c, &c->adj[q][cc]));
-      HIPCHK(c, hipMemsetAsync(c->adj[q][cc], 0, bytes, c->stream));
+  for (int q = 0; q < 4 && e == hipSuccess; q++)
+    for (int cc = 0; cc < 3 && e == hipSuccess; cc++) {
+      e = alloc_field_storage(c, &adj[q][cc]);
+      if (e == hipSuccess) e = hipMemsetAsync(adj[q][cc], 0, bytes, c->stream);
     }
-  HIPCHK(c, hipMalloc(&c->adj_partial, 3 * kAdjMaxBlocks * sizeof(double)));
+  if (e == hipSuccess) e = hipMalloc(&partial, 3 * kAdjMaxBlocks * sizeof(double));
   // [0..2]: the force adjoint of the last step; [3..5]: the accumulator of
   // the steps called without a host force_bar (mif_adjoint_force_take)
-  HIPCHK(c, hipMalloc(&c->adj_fbar, 6 * sizeof(double)));
-  HIPCHK(c, hipMemsetAsync(c->adj_fbar, 0, 6 * sizeof(double), c->stream));
+  if (e == hipSuccess) e = hipMalloc(&fbar, 6 * sizeof(double));
+  if (e == hipSuccess) e = hipMemsetAsync(fbar, 0, 6 * sizeof(double), c->stream);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& row : adj)
+      for (void* p : row)
+        if (p) free_field_storage(c, p);
+    (void)hipFree(partial);
+    (void)hipFree(fbar);
+    return fail(c, e == hipErrorOutOfMemory ? MIF_ERR_OUT_OF_MEMORY : MIF_ERR_HIP, "adjoint workspace: %s",
+                hipGetErrorString(e));
+  }
+  std::memcpy(c->adj, adj, sizeof(adj));
+  c->adj_partial = partial;
+  c->adj_fbar = fbar;
+  c->adj_ready = true;
   return MIF_OK;
 }
 
@@ -745,8 +867,64 @@ int step_adjoint(mif_ctx* c, mif_field* const un[3], mif_field* const y2[3], mif
   return MIF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Boundary data evaluated on the host (callbacks with no closed form).  A
+// velocity face table holds, for u, v, w in turn, the faces z-, z+, y-, y+,
+// x-, x+ of non-periodic axes, each a plane over the component's extents
+// along the two other axes (lower axis fastest) with the value
+// VelocityTensor::apply_bc writes there (src/VelocityTensor.cpp:36-233).  A
+// Neumann table holds the six wall faces of the node extents with the
+// normal component of the pressure-gradient data at each wall node
+// (apply_non_homogeneous_neumann, src/PressureEquation.cpp:10-57).
+// ---------------------------------------------------------------------------
+long face_table_layout(const DGrid& g, bool velocity, long toff[3][6]) {
+  long n = 0;
+  for (int q = 0; q < 3; q++)
+    for (int f = 0; f < 6; f++) toff[q][f] = -1;
+  for (int q = 0; q < (velocity ? 3 : 1); q++) {
+    int e[3];
+    extents(g, velocity ? q : MIF_STAGGER_NONE, e);
+    for (int f = 0; f < 6; f++) {
+      const int axis = 2 - (f >> 1);
+      if (g.periodic[axis]) continue;
+      toff[q][f] = n;
+      n += axis == 2 ? (long)e[0] * e[1] : (axis == 1 ? (long)e[0] * e[2] : (long)e[1] * e[2]);
+    }
+  }
+  return n;
+}
+
+// Copies `count` host tables of `velocity` (or Neumann) layout into the
+// context's table buffer (allocated on first use) and returns their
+// descriptors in out[0..count).  The copy is ordered after the work already
+// on the stream and complete at return, so the caller may reuse its arrays.
+int upload_face_tables(mif_ctx* c, bool velocity, const double* host, int count, FnDesc* out) {
+  long vtoff[3][6], ntoff[3][6];
+  const long nv = face_table_layout(c->g, true, vtoff);
+  const long nn = face_table_layout(c->g, false, ntoff);
+  if (!c->ftab) {
+    c->ftab_elems = 3 * (nv + nn);
+    HIPCHK(c, hipMalloc(&c->ftab, std::max<long>(c->ftab_elems, 1) * sizeof(double)));
+  }
+  const long per = velocity ? nv : nn;
+  double* dst = c->ftab + (velocity ? 0 : 3 * nv);
+  if (per > 0) {
+    HIPCHK(c, hipMemcpyAsync(dst, host, count * per * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  for (int q = 0; q < count; q++) {
+    FnDesc d{};
+    d.kind = kFnFaceTable;
+    d.tab = dst + q * per;
+    std::memcpy(d.toff, velocity ? vtoff : ntoff, sizeof(d.toff));
+    out[q] = d;
+  }
+  return MIF_OK;
+}
+
 // Constants, layout, transform plans and scratch of a new context.
-int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precision, int device) {
+int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precision, int device,
+             const int cell[3]) {
   if (device < 0) (void)hipGetDevice(&device);
   c->device = device;
   if (hipSetDevice(device) != hipSuccess) return fail(nullptr, MIF_ERR_NO_DEVICE, "hipSetDevice(%d) failed", device);
@@ -765,7 +943,9 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
     g.periodic[a] = grid->periodic[a] ? 1 : 0;
     g.N[a] = g.Ng[a] + g.periodic[a];
     g.base[a] = g.periodic[a] ? -1 : 0;
-    g.npts[a] = g.Ng[a] - g.periodic[a];
+    g.cell[a] = cell && cell[a] ? 1 : 0;
+    // cell-centred axis: the N_g - 1 cells between the nodes
+    g.npts[a] = g.Ng[a] - (g.periodic[a] || g.cell[a] ? 1 : 0);
     g.olo[a] = g.periodic[a];
     const double h = size[a] / (double)(Ng[a] - 1);
     g.h[a] = h;
@@ -834,6 +1014,10 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
   if (precision == MIF_PRECISION_MIXED) {
     HIPCHK(nullptr, hipMalloc(&c->B64, c->belems * sizeof(double)));
     HIPCHK(nullptr, hipMalloc(&c->P64, c->belems * sizeof(double)));
+    HIPCHK(nullptr, hipMalloc(&c->B2, c->belems * sizeof(float)));
+    HIPCHK(nullptr, hipMemsetAsync(c->B2, 0, c->belems * sizeof(float), c->stream));
+    HIPCHK(nullptr, hipMalloc(&c->red_b, 2 * sizeof(Red3)));
+    c->red_phi = c->red_b + 1;
   }
   double norm = 1.0;
   double lam_max = 0, lam_min = 0;  // |lambda| extremes of the 3-D operator
@@ -842,8 +1026,27 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
     FftPlan& P = c->plan[a];
     P.n = n;
     P.periodic = g.periodic[a];
-    P.L = g.periodic[a] ? n : 2 * (n - 1);
+    P.dct2 = g.cell[a];
+    P.L = g.periodic[a] || g.cell[a] ? n : 2 * (n - 1);
     factor_radices(P.L, P);
+    if (P.dct2) {
+      // (cos, sin)(pi k / (2n)) of the DCT-II/III pre- and post-processing
+      std::vector<double> h64(2 * (size_t)n);
+      std::vector<float> h32(2 * (size_t)n);
+      for (int k = 0; k < n; k++) {
+        const long double ang = 3.141592653589793238462643383279502884L * (long double)k / (2.0L * n);
+        h64[2 * k] = (double)cosl(ang);
+        h64[2 * k + 1] = (double)sinl(ang);
+        h32[2 * k] = (float)cosl(ang);
+        h32[2 * k + 1] = (float)sinl(ang);
+      }
+      HIPCHK(nullptr, hipMalloc(&c->H64[a], h64.size() * sizeof(double)));
+      HIPCHK(nullptr, hipMalloc(&c->H32[a], h32.size() * sizeof(float)));
+      HIPCHK(nullptr, hipMemcpy(c->H64[a], h64.data(), h64.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(nullptr, hipMemcpy(c->H32[a], h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice));
+      P.H64 = c->H64[a];
+      P.H32 = c->H32[a];
+    }
     std::vector<double> w64(2 * (size_t)P.L);
     std::vector<float> w32(2 * (size_t)P.L);
     for (int e = 0; e < P.L; e++) {
@@ -861,7 +1064,8 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
     P.W32 = c->W32[a];
     std::vector<double> lam(n);
     for (int k = 0; k < n; k++)
-      lam[k] = g.periodic[a] ? eig_periodic(k, g.h[a], n) : eig_neumann(k, g.h[a], n);
+      lam[k] = g.periodic[a] ? eig_periodic(k, g.h[a], n)
+                             : (g.cell[a] ? eig_cell_neumann(k, g.h[a], n) : eig_neumann(k, g.h[a], n));
     HIPCHK(nullptr, hipMalloc(&c->lam[a], n * sizeof(double)));
     HIPCHK(nullptr, hipMemcpy(c->lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
     double amax = 0, amin = 0;
@@ -872,11 +1076,16 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
     }
     lam_max += amax;
     if (amin > 0 && (lam_min == 0 || amin < lam_min)) lam_min = amin;
-    // normalisation after each inverse axis (src/PressureEquation.cpp:167,200,234)
-    norm *= (double)(Ng[a] - 1) * (g.periodic[a] ? 1.0 : 2.0);
+    // normalisation after each inverse axis (src/PressureEquation.cpp:167,200,234);
+    // the DCT-III of a cell-centred axis returns n times the line
+    norm *= (double)(Ng[a] - 1) * (g.periodic[a] || g.cell[a] ? 1.0 : 2.0);
   }
   c->inv_norm = 1.0 / norm;
   c->kappa = lam_min > 0 ? lam_max / lam_min : 0;
+  if (precision == MIF_PRECISION_MIXED) {
+    c->nxpart = fused_x_blocks(g, c->plan[0], false);
+    if (c->nxpart > 0) HIPCHK(nullptr, hipMalloc(&c->xpart, c->nxpart * sizeof(Red3)));
+  }
   // transfers and reductions
   long mx = 0;
   for (int s = 0; s < 4; s++) {
@@ -904,7 +1113,45 @@ extern "C" {
 
 const char* mif_last_error(const mif_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
+static int ctx_create(const mif_grid* grid, int precision, int device, const int cell[3], mif_ctx** out);
+
 int mif_ctx_create(const mif_grid* grid, int precision, int device, mif_ctx** out) {
+  return ctx_create(grid, precision, device, nullptr, out);
+}
+
+int mif_ctx_create_cell_centred(const mif_grid* grid, const int cell_centred[3], int precision, int device,
+                                mif_ctx** out) {
+  if (!grid || !cell_centred || !out) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
+  for (int a = 0; a < 3; a++) {
+    if (!cell_centred[a]) continue;
+    if (grid->periodic[a])
+      return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "axis %d: a cell-centred axis has walls, not periodic ends", a);
+    if (a != 1)
+      return fail(nullptr, MIF_ERR_UNSUPPORTED, "axis %d: cell-centred transforms run on the y axis only", a);
+    const uint64_t n = grid->Ny_global - 1;
+    if (n != 64 && n != 128 && n != 256 && n != 512 && n != 1024)
+      return fail(nullptr, MIF_ERR_UNSUPPORTED,
+                  "cell-centred y axis of %llu cells: 64, 128, 256, 512 or 1024 cells are supported",
+                  (unsigned long long)n);
+  }
+  return ctx_create(grid, precision, device, cell_centred, out);
+}
+
+// LDS bytes one workgroup of the generic transform kernel needs on an axis
+// (two line buffers of the complex length, one line pair, FP64 so that the
+// mixed form's FP64 transforms fit too); 0 where a specialised kernel runs.
+static size_t generic_transform_lds(int axis, uint64_t Ng, bool periodic, bool cell) {
+  if (cell) return 0;
+  const long n = (long)Ng - (periodic ? 1 : 0);
+  const long L = periodic ? n : 2 * (n - 1);
+  if (L < 2 || fft_fast_len((int)L)) return 0;
+  // the half-length DCT-I of 513, 769 and 1025-node wall-normal y axes
+  if (axis == 1 && !periodic && (L / 2 == 512 || L / 2 == 768 || L / 2 == 1024)) return 0;
+  return 2 * (size_t)L * 2 * sizeof(double);
+}
+constexpr size_t kGenericTransformLdsMax = 64 * 1024;
+
+static int ctx_create(const mif_grid* grid, int precision, int device, const int cell[3], mif_ctx** out) {
   if (!grid || !out) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
   if (precision != MIF_PRECISION_FP64 && precision != MIF_PRECISION_MIXED)
@@ -918,11 +1165,19 @@ int mif_ctx_create(const mif_grid* grid, int precision, int device, mif_ctx** ou
   if (grid->num_time_steps == 0 || !(grid->final_time > 0))
     return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "final_time and num_time_steps must be positive");
   if (!(grid->Re > 0)) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "Re must be positive");
+  for (int a = 0; a < 3; a++) {
+    const size_t lds = generic_transform_lds(a, Ng[a], grid->periodic[a] != 0, cell && cell[a]);
+    if (lds > kGenericTransformLdsMax)
+      return fail(nullptr, MIF_ERR_UNSUPPORTED,
+                  "axis %d: %llu nodes give a transform length with no specialised kernel whose generic kernel "
+                  "needs %zu bytes of LDS per workgroup (limit %zu)",
+                  a, (unsigned long long)Ng[a], lds, kGenericTransformLdsMax);
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, MIF_ERR_NO_DEVICE, "no HIP device");
   mif_ctx* c = new (std::nothrow) mif_ctx();
   if (!c) return fail(nullptr, MIF_ERR_OUT_OF_MEMORY, "host allocation failed");
-  const int rc = ctx_init(c, grid, Ng, precision, device);
+  const int rc = ctx_init(c, grid, Ng, precision, device, cell);
   if (rc != MIF_OK) {
     mif_ctx_destroy(c);
     return rc;
@@ -949,15 +1204,21 @@ int mif_ctx_destroy(mif_ctx* c) {
   (void)hipFree(c->B);
   (void)hipFree(c->B64);
   (void)hipFree(c->P64);
+  (void)hipFree(c->B2);
+  (void)hipFree(c->xpart);
+  (void)hipFree(c->red_b);
   (void)hipFree(c->R64);
   for (int q = 0; q < 4; q++)
     for (int cc = 0; cc < 3; cc++) free_field_storage(c, c->adj[q][cc]);
+  (void)hipFree(c->ftab);
   (void)hipFree(c->adj_partial);
   (void)hipFree(c->adj_fbar);
   (void)hipFree(c->trig);
   for (int a = 0; a < 3; a++) {
     (void)hipFree(c->W64[a]);
     (void)hipFree(c->W32[a]);
+    (void)hipFree(c->H64[a]);
+    (void)hipFree(c->H32[a]);
     (void)hipFree(c->lam[a]);
   }
   (void)hipFree(c->staging);
@@ -1013,6 +1274,10 @@ int mif_ctx_get_defect_sweeps(const mif_ctx* c, int* sweeps) {
 int mif_field_create(mif_ctx* c, int stag, mif_field** out) {
   if (!c || !out) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null argument");
   if (stag < 0 || stag > 3) return fail(c, MIF_ERR_INVALID_ARGUMENT, "unknown staggering %d", stag);
+  if (c->g.cell[0] || c->g.cell[1] || c->g.cell[2])
+    return fail(c, MIF_ERR_UNSUPPORTED,
+                "a context with a cell-centred axis runs stand-alone Poisson solves only "
+                "(mif_poisson_set_rhs / mif_poisson_run / mif_poisson_get_solution); it has no staggered fields");
   mif_field* f = new (std::nothrow) mif_field{c, stag, nullptr};
   if (!f) return fail(c, MIF_ERR_OUT_OF_MEMORY, "host allocation failed");
   hipError_t e = alloc_field_storage(c, &f->data);
@@ -1165,43 +1430,134 @@ int mif_velocity_apply_bc(mif_field* const vel[3], const mif_function* fn, doubl
   return MIF_OK;
 }
 
-static int timestep_common(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
-                           const mif_function* ev, const mif_function* gp, double t_n, mif_field* p, mif_field* pb) {
+int mif_bc_table_size(const mif_ctx* c, int64_t* velocity_faces, int64_t* neumann_faces) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  long toff[3][6];
+  if (velocity_faces) *velocity_faces = face_table_layout(c->g, true, toff);
+  if (neumann_faces) *neumann_faces = face_table_layout(c->g, false, toff);
+  return MIF_OK;
+}
+
+int mif_velocity_apply_bc_tabulated(mif_field* const vel[3], const double* faces) {
+  mif_ctx* c = nullptr;
+  int rc = check_vel(vel, &c);
+  if (rc) return rc;
+  if (!faces) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null face table");
+  FnDesc d;
+  if ((rc = upload_face_tables(c, true, faces, 1, &d))) return rc;
+  if (is_fp64(c))
+    apply_bc<double>(c, vel, d, 0.0);
+  else
+    apply_bc<float>(c, vel, d, 0.0);
+  LAUNCHCHK(c);
+  return MIF_OK;
+}
+
+// Argument checks of the timestep entry points; c receives the context.
+static int check_timestep(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3], mif_field* p,
+                          mif_field* pb, bool nhn, mif_ctx** out) {
   mif_ctx *c = nullptr, *c2 = nullptr, *c3 = nullptr;
   int rc;
   if ((rc = check_vel(vel, &c)) || (rc = check_vel(buf, &c2)) || (rc = check_vel(buf2, &c3))) return rc;
-  if (c2 != c || c3 != c || !p || !pb || p->ctx != c || pb->ctx != c || !ev)
+  if (c2 != c || c3 != c || !p || !pb || p->ctx != c || pb->ctx != c)
     return fail(c, MIF_ERR_INVALID_ARGUMENT, "inconsistent timestep arguments");
   if (p->stag != MIF_STAGGER_NONE || pb->stag != MIF_STAGGER_NONE)
     return fail(c, MIF_ERR_INVALID_ARGUMENT, "pressure fields must be unstaggered");
-  if (gp && (c->g.periodic[0] || c->g.periodic[1] || c->g.periodic[2]))
+  if (nhn && (c->g.periodic[0] || c->g.periodic[1] || c->g.periodic[2]))
     return fail(c, MIF_ERR_INVALID_ARGUMENT, "non-homogeneous Neumann data needs walls on every axis");
-  const FnDesc d = make_fn(ev);
-  if (is_fp64(c)) return full_timestep<double>(c, vel, buf, buf2, d, gp, t_n, p, pb);
-  return full_timestep<float>(c, vel, buf, buf2, d, gp, t_n, p, pb);
+  *out = c;
+  return MIF_OK;
+}
+
+static int run_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
+                        const FnDesc ev[3], const FnDesc* gd, double t_n, mif_field* p, mif_field* pb) {
+  if (is_fp64(c)) return full_timestep<double>(c, vel, buf, buf2, ev, gd, t_n, p, pb);
+  return full_timestep<float>(c, vel, buf, buf2, ev, gd, t_n, p, pb);
 }
 
 int mif_timestep(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
                  const mif_function* exact_velocity, double t_n, mif_field* p, mif_field* pb) {
-  return timestep_common(vel, buf, buf2, exact_velocity, nullptr, t_n, p, pb);
+  mif_ctx* c = nullptr;
+  int rc = check_timestep(vel, buf, buf2, p, pb, false, &c);
+  if (rc) return rc;
+  if (!exact_velocity) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null exact velocity");
+  const FnDesc d = make_fn(exact_velocity);
+  const FnDesc ev[3] = {d, d, d};
+  return run_timestep(c, vel, buf, buf2, ev, nullptr, t_n, p, pb);
 }
 
 int mif_timestep_nhn(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
                      const mif_function* exact_velocity, const mif_function* exact_grad_p, double t_n, mif_field* p,
                      mif_field* pb) {
-  if (!exact_grad_p) return fail(vel && vel[0] ? vel[0]->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null gradient");
-  return timestep_common(vel, buf, buf2, exact_velocity, exact_grad_p, t_n, p, pb);
+  mif_ctx* c = nullptr;
+  int rc = check_timestep(vel, buf, buf2, p, pb, true, &c);
+  if (rc) return rc;
+  if (!exact_velocity || !exact_grad_p) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null function");
+  const FnDesc d = make_fn(exact_velocity);
+  const FnDesc ev[3] = {d, d, d};
+  // TimeVectorFunction::get_difference_over_time(new_time, prev_time) yields
+  // f(prev_time) - f(new_time) (src/Timestep.cpp:89-95, src/VectorFunction.cpp:53-61)
+  const StageTimes st = stage_times(c->g.dt, t_n);
+  FnDesc gd[3];
+  for (int s = 0; s < 3; s++) {
+    gd[s] = make_fn(exact_grad_p);
+    gd[s].base = exact_grad_p->kind;
+    gd[s].kind = MIF_FN_TIME_DIFFERENCE;
+    gd[s].t0 = st.time[s + 1];
+    gd[s].t1 = st.time[s];
+  }
+  return run_timestep(c, vel, buf, buf2, ev, gd, t_n, p, pb);
+}
+
+int mif_timestep_tabulated(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
+                           const double* faces, double t_n, mif_field* p, mif_field* pb) {
+  mif_ctx* c = nullptr;
+  int rc = check_timestep(vel, buf, buf2, p, pb, false, &c);
+  if (rc) return rc;
+  if (!faces) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null face tables");
+  FnDesc ev[3];
+  if ((rc = upload_face_tables(c, true, faces, 3, ev))) return rc;
+  return run_timestep(c, vel, buf, buf2, ev, nullptr, t_n, p, pb);
+}
+
+int mif_timestep_nhn_tabulated(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
+                               const double* faces, const double* neumann, double t_n, mif_field* p, mif_field* pb) {
+  mif_ctx* c = nullptr;
+  int rc = check_timestep(vel, buf, buf2, p, pb, true, &c);
+  if (rc) return rc;
+  if (!faces || !neumann) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null tables");
+  FnDesc ev[3], gd[3];
+  if ((rc = upload_face_tables(c, true, faces, 3, ev))) return rc;
+  if ((rc = upload_face_tables(c, false, neumann, 3, gd))) return rc;
+  return run_timestep(c, vel, buf, buf2, ev, gd, t_n, p, pb);
+}
+
+static int velocity_timestep_any(mif_field* const vel[3], mif_field* const buf[3], mif_field* const rhs[3],
+                                 const mif_function* ev, const double* faces, const mif_function* fo, double t_n) {
+  mif_ctx *c = nullptr, *c2 = nullptr, *c3 = nullptr;
+  int rc;
+  if ((rc = check_vel(vel, &c)) || (rc = check_vel(buf, &c2)) || (rc = check_vel(rhs, &c3))) return rc;
+  if (c2 != c || c3 != c || !(ev || faces) || !fo) return fail(c, MIF_ERR_INVALID_ARGUMENT, "inconsistent arguments");
+  FnDesc d[3];
+  if (faces) {
+    if ((rc = upload_face_tables(c, true, faces, 3, d))) return rc;
+  } else {
+    d[0] = d[1] = d[2] = make_fn(ev);
+  }
+  const FnDesc f = make_fn(fo);
+  if (is_fp64(c)) return velocity_timestep<double>(c, vel, buf, rhs, d, f, t_n);
+  return velocity_timestep<float>(c, vel, buf, rhs, d, f, t_n);
 }
 
 int mif_timestep_velocity(mif_field* const vel[3], mif_field* const buf[3], mif_field* const rhs[3],
                           const mif_function* ev, const mif_function* fo, double t_n) {
-  mif_ctx *c = nullptr, *c2 = nullptr, *c3 = nullptr;
-  int rc;
-  if ((rc = check_vel(vel, &c)) || (rc = check_vel(buf, &c2)) || (rc = check_vel(rhs, &c3))) return rc;
-  if (c2 != c || c3 != c || !ev || !fo) return fail(c, MIF_ERR_INVALID_ARGUMENT, "inconsistent arguments");
-  const FnDesc d = make_fn(ev), f = make_fn(fo);
-  if (is_fp64(c)) return velocity_timestep<double>(c, vel, buf, rhs, d, f, t_n);
-  return velocity_timestep<float>(c, vel, buf, rhs, d, f, t_n);
+  return velocity_timestep_any(vel, buf, rhs, ev, nullptr, fo, t_n);
+}
+
+int mif_timestep_velocity_tabulated(mif_field* const vel[3], mif_field* const buf[3], mif_field* const rhs[3],
+                                    const double* faces, const mif_function* fo, double t_n) {
+  if (!faces) return fail(vel && vel[0] ? vel[0]->ctx : nullptr, MIF_ERR_INVALID_ARGUMENT, "null face tables");
+  return velocity_timestep_any(vel, buf, rhs, nullptr, faces, fo, t_n);
 }
 
 int mif_solve_pressure_homogeneous_periodic(mif_field* p, mif_field* const vel[3], double dt) {
@@ -1210,7 +1566,8 @@ int mif_solve_pressure_homogeneous_periodic(mif_field* p, mif_field* const vel[3
   if (rc) return rc;
   if (!p || p->ctx != c || p->stag != MIF_STAGGER_NONE) return fail(c, MIF_ERR_INVALID_ARGUMENT, "bad pressure field");
   FnDesc none{};
-  return pressure_solve_any(c, p, vel, dt, none, 0.0, 0);
+  if ((rc = pressure_solve_any(c, p, vel, dt, none, 0.0, 0))) return rc;
+  return finish_standalone_solve(c, p);
 }
 
 int mif_solve_pressure_non_homogeneous_neumann(mif_field* p, mif_field* const vel[3], const mif_function* gp,
@@ -1224,6 +1581,20 @@ int mif_solve_pressure_non_homogeneous_neumann(mif_field* p, mif_field* const ve
     return fail(c, MIF_ERR_INVALID_ARGUMENT, "non-homogeneous Neumann data needs walls on every axis");
   const FnDesc d = make_fn(gp);
   return pressure_solve_any(c, p, vel, dt, d, t, 1);
+}
+
+int mif_solve_pressure_non_homogeneous_neumann_tabulated(mif_field* p, mif_field* const vel[3], const double* neumann,
+                                                         double dt) {
+  mif_ctx* c = nullptr;
+  int rc = check_vel(vel, &c);
+  if (rc) return rc;
+  if (!p || p->ctx != c || p->stag != MIF_STAGGER_NONE || !neumann)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "bad pressure field or Neumann table");
+  if (c->g.periodic[0] || c->g.periodic[1] || c->g.periodic[2])
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "non-homogeneous Neumann data needs walls on every axis");
+  FnDesc d;
+  if ((rc = upload_face_tables(c, false, neumann, 1, &d))) return rc;
+  return pressure_solve_any(c, p, vel, dt, d, 0.0, 1);
 }
 
 int mif_poisson_solve(mif_field* p) {

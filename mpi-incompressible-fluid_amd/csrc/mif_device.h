@@ -24,6 +24,7 @@ struct DGrid {
   int periodic[3];
   int npts[3];     // Poisson unknowns per axis
   int olo[3];      // first owner index per axis (1 on periodic axes)
+  int cell[3];     // 1: cell-centred Neumann axis (stand-alone Poisson solves)
   double h[3], min[3];
   double one_over_2h[3], one_over_8h[3], one_over_h2Re[3], h_over_2[3], one_over_h[3];
   double Re, dt;
@@ -54,10 +55,30 @@ __host__ __device__ inline void extents(const DGrid& g, int s, int e[3]) {
 // Analytic functions (device closed forms of the reference's generated
 // callbacks; see include/mif_types.h for the list).
 // ---------------------------------------------------------------------------
+// kind kFnFaceTable: boundary data evaluated on the host, for callbacks that
+// have no closed form here.  tab holds, for each component q (of a velocity;
+// only q = 0 for the Neumann data of a node field) and each face
+// f = z-, z+, y-, y+, x-, x+, a plane of values over the field's extents
+// along the two other axes, lower axis fastest (face_plane_index), starting
+// at tab + toff[q][f]; faces of periodic axes have none.  fn_eval never
+// reads it; the boundary and Neumann-data kernels look values up instead.
+constexpr int kFnFaceTable = 1000;
+
 struct FnDesc {
   int kind, base;
   double Re, c[3], t0, t1;
+  const double* tab;
+  long toff[3][6];
 };
+
+// Index of point (i, j, k) in the plane of face f (0,1 = z-,z+; 2,3 =
+// y-,y+; 4,5 = x-,x+) of a field with extents e.
+__host__ __device__ inline long face_plane_index(int face, const int e[3], int i, int j, int k) {
+  const int axis = 2 - (face >> 1);
+  if (axis == 2) return (long)i + (long)e[0] * j;
+  if (axis == 1) return (long)i + (long)e[0] * k;
+  return (long)j + (long)e[1] * k;
+}
 
 __host__ inline FnDesc make_fn(const mif_function* f) {
   FnDesc d{};
@@ -175,6 +196,14 @@ __device__ inline double coord(const DGrid& g, int a, int s, int i) {
 
 __device__ inline double eval_at(const DGrid& g, const FnDesc& f, int s, int c, double t, int i, int j, int k) {
   return fn_eval(f, c, t, coord(g, 0, s, i), coord(g, 1, s, j), coord(g, 2, s, k));
+}
+
+// Weight of owner index i along axis a in the left null vector of the 1-D
+// operator the solve inverts: uniform on periodic and cell-centred axes,
+// trapezoid (1/2 at the two end nodes) on node Neumann axes.
+__device__ inline double owner_weight(const DGrid& g, int a, int i) {
+  if (g.periodic[a] || g.cell[a]) return 1.0;
+  return (i == 0 || i == g.npts[a] - 1) ? 0.5 : 1.0;
 }
 
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs of the

@@ -384,6 +384,12 @@ template <typename T>
 void launch_fft_axis(const DGrid& g, int axis, int dir, int fused, const FftPlan& plan, const SpectralScale& sc, T* B,
                      long bp0, long bplane, hipStream_t st) {
   const int mode = fused ? 2 : (dir < 0 ? 0 : 1);
+  if (plan.dct2) {
+    // cell-centred axes: y only, lengths with a kernel (checked when the
+    // context is created)
+    (void)launch_dct2_y_any<T>(g, plan, dir > 0, B, bp0, bplane, st);
+    return;
+  }
   if (plan.L >= 2 && launch_fft_pow2<T>(g, axis, mode, plan, sc, B, bp0, bplane, nullptr, st)) return;
   if (axis == 0) {
     if (mode == 0) launch_one<T, 0, 0>(g, plan, sc, B, bp0, bplane, st);

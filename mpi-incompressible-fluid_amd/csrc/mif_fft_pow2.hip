@@ -240,9 +240,12 @@ using p2::pd;
 // periodic ghost images (PressureTensor::copy_to_staggered,
 // src/PressureTensor.cpp:21-34), instead of B.
 // TV: element type of the velocity and pressure fields of the fused passes.
-template <typename T, int L, int NP, int AXIS, int MODE, bool FUSED, typename TV = T>
+// DC (fused x passes of the mixed form's one-sweep defect correction, FP32):
+// see FusedIO::dc.
+template <typename T, int L, int NP, int AXIS, int MODE, bool FUSED, typename TV = T, bool DC = false>
 __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W, SpectralScale sc, T* __restrict__ B,
                                                 long bp0, long bplane, int periodic, FusedIO io) {
+  static_assert(!DC || (FUSED && AXIS == 0), "defect-correction outputs belong to the fused x passes");
   // twiddles live in LDS when the workgroup stays within MIF_TWL_MAX
   // bytes, otherwise they are read from the (L1-resident) global table
   constexpr int LP = p2::lds_pitch(L);
@@ -330,6 +333,12 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   // (an element outside the lines reads index 0 and is discarded), so a
   // chunk's loads are in flight together before its values go to LDS.
   constexpr int R0 = (L + 255) / 256;  // elements per thread and line on axis 0
+  // DC: this thread's owner-weighted sums (W.b or W.phi, W.1)
+  double dc_s = 0, dc_w = 0;
+  auto line_weight = [&](int l) -> double {
+    const int r = bx * NL + l;
+    return owner_weight(g, 1, r % n1) * owner_weight(g, 2, r / n1);
+  };
   if (AXIS == 0 && FUSED && MODE == 0) {
     const TV* u = (const TV*)io.u;
     const TV* v = (const TV*)io.v;
@@ -353,7 +362,16 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
           const double du_dx = ((double)u1 - (double)u0) * g.one_over_h[0];
           const double dv_dy = ((double)v1 - (double)v0) * g.one_over_h[1];
           const double dw_dz = ((double)w1 - (double)w0) * g.one_over_h[2];
-          val[a][q] = okm ? (T)((du_dx + dv_dy + dw_dz) / io.dt) : (T)0;
+          const double bv = (du_dx + dv_dy + dw_dz) / io.dt;
+          val[a][q] = okm ? (T)bv : (T)0;
+          if constexpr (DC) {
+            if (okm) {
+              io.b64[lbase[l] + m] = bv;
+              const double wgt = owner_weight(g, 0, m) * line_weight(l);
+              dc_s += wgt * bv;
+              dc_w += wgt;
+            }
+          }
         }
       }
 #pragma unroll
@@ -509,7 +527,19 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
 #pragma unroll
         for (int l = 0; l < NL; l++) {
           const Cx<T> a = Z[l % NP][pd(m)];
-          val[l] = (TV)(l < NP ? a.re : a.im);
+          if constexpr (DC) {
+            // phi = phi_0 + delta in FP64, stored in the field's precision
+            const long b = lbase[l];
+            const double phi = (double)(l < NP ? a.re : a.im) + (b >= 0 ? (double)io.add32[b + m] : 0.0);
+            val[l] = (TV)phi;
+            if (b >= 0) {
+              const double wgt = owner_weight(g, 0, m) * line_weight(l);
+              dc_s += wgt * (double)val[l];
+              dc_w += wgt;
+            }
+          } else {
+            val[l] = (TV)(l < NP ? a.re : a.im);
+          }
         }
 #pragma unroll
         for (int l = 0; l < NL; l++)
@@ -540,6 +570,22 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
           MIF_ST_F(&B[out_index(l, m)], l < NP ? a.re : a.im);
         }
     }
+  }
+  if constexpr (DC) {
+    // the workgroup's (W.x, W.1) into its slot of io.partial
+    for (int off = 32; off > 0; off >>= 1) {
+      dc_s += __shfl_down(dc_s, off, 64);
+      dc_w += __shfl_down(dc_w, off, 64);
+    }
+    __shared__ double red[2][4];
+    if ((t & 63) == 0) {
+      red[0][t >> 6] = dc_s;
+      red[1][t >> 6] = dc_w;
+    }
+    __syncthreads();
+    if (t == 0)
+      io.partial[bx + gridDim.x * by] = Red3{red[0][0] + red[0][1] + red[0][2] + red[0][3],
+                                              red[1][0] + red[1][1] + red[1][2] + red[1][3], 0};
   }
 }
 
@@ -722,6 +768,172 @@ __global__ void __launch_bounds__(256) k_dct1_y(DGrid g, const T* __restrict__ W
   }
 }
 
+// Cell-centred Neumann y axis (no counterpart in the reference, whose
+// Neumann axes are node-centred DCT-I): N unknowns at the cell centres, the
+// ghost reflection phi_{-1} = phi_0, phi_N = phi_{N-1}, so the 1-D operator
+// has rows (-1, 1), (1, -2, 1), (1, -1) over h^2.  Its eigenvectors are
+// cos(pi k (j + 1/2) / N) with eigenvalues 2 (cos(pi k / N) - 1) / h^2, so
+// the transform pair is the DCT-II
+//   X_k = sum_j x_j cos(pi k (2j + 1) / (2N)),           k = 0..N-1,
+// and the DCT-III
+//   y_j = X_0 + 2 sum_{k>=1} X_k cos(pi k (2j + 1) / (2N)) = N x_j.
+// Both go through one N-point complex FFT per pair of lines (Makhoul's
+// reordering): v_m = x_{2m}, v_{N-1-m} = x_{2m+1} (m < N/2), V = DFT(v),
+// X_k = Re(exp(-i pi k / (2N)) V_k); backwards V_k = exp(i pi k / (2N))
+// (X_k - i X_{N-k}) with X_N = 0, and the unnormalised inverse DFT of V is N
+// times v.  Two lines a, b share the FFT as v^a + i v^b.  H holds
+// (cos, sin)(pi k / (2N)), k < N.
+template <typename T, int N, int NP, int INV>
+__global__ void __launch_bounds__(256) k_dct2_y(DGrid g, const T* __restrict__ W, const T* __restrict__ H,
+                                                T* __restrict__ B, long bp0, long bplane) {
+  constexpr int LP = p2::lds_pitch(N);
+  constexpr int NL = 2 * NP;
+  constexpr int STEP = 256 / NL;
+  constexpr int MQ = (N + STEP - 1) / STEP;
+  constexpr int TWN = p2::tw_count(N);
+  static_assert((NP * LP + TWN + N) * sizeof(Cx<T>) <= MIF_LDS_MAX, "workgroup LDS above the limit");
+  constexpr int NI = NP * N;  // post/pre items: (pair, k)
+  constexpr int PER = (NI + 255) / 256;
+  __shared__ Cx<T> Z[NP][LP];
+  __shared__ Cx<T> tws[TWN];  // exp(-2 pi i e / N), e < TWN
+  __shared__ Cx<T> hc[N];     // (cos, sin)(pi k / (2N))
+  const int t = threadIdx.x;
+  int bx, by;
+  {
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    bx = tile % gridDim.x;
+    by = tile / gridDim.x;
+  }
+  for (int e = t; e < TWN; e += 256) tws[e] = Cx<T>{W[2 * e], W[2 * e + 1]};
+  for (int k = t; k < N; k += 256) hc[k] = Cx<T>{H[2 * k], H[2 * k + 1]};
+  // line l: pair l % NP, real part (l < NP) or imaginary part; thread t
+  // serves line t % NL, rows t / NL + q * STEP
+  const int l = t % NL;
+  const int i = bx * NL + l;
+  const long b = i < g.npts[0] ? i + bplane * by : -1;
+  auto line = [&](int ll) -> T* { return (T*)&Z[ll % NP][0] + (ll < NP ? 0 : LP); };
+  T val[MQ];
+#pragma unroll
+  for (int q = 0; q < MQ; q++) {
+    const int m = t / NL + q * STEP;
+    const bool okm = b >= 0 && m < N;
+    const T x = B[okm ? b + m * bp0 : 0];
+    val[q] = okm ? x : (T)0;
+  }
+  __syncthreads();  // twiddle tables staged
+  if (!INV) {
+    // x_j -> v at the reordered position
+#pragma unroll
+    for (int q = 0; q < MQ; q++) {
+      const int j = t / NL + q * STEP;
+      if (j < N) {
+        const int m = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+        ((T*)&Z[l % NP][pd(m)])[l < NP ? 0 : 1] = val[q];
+      }
+    }
+    __syncthreads();
+    p2::fft<T, N, NP, -1>(Z, tws);
+    // X_k of both lines, straight to the packed buffer
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int it = t + 256 * r;
+      if (NI % 256 == 0 || it < NI) {
+        const int p = it % NP, k = it / NP;
+        const Cx<T> A = Z[p][pd(k)], Bk = Z[p][pd(k == 0 ? 0 : N - k)];
+        const T c = hc[k].re, sn = hc[k].im;
+        const T xa = c * (A.re + Bk.re) * (T)0.5 + sn * (A.im - Bk.im) * (T)0.5;
+        const T xb = c * (A.im + Bk.im) * (T)0.5 + sn * (Bk.re - A.re) * (T)0.5;
+        const long b1 = bx * NL + p < g.npts[0] ? bx * NL + p + bplane * by : -1;
+        const long b2 = bx * NL + p + NP < g.npts[0] ? bx * NL + p + NP + bplane * by : -1;
+        if (b1 >= 0) MIF_ST_F(&B[b1 + k * bp0], xa);
+        if (b2 >= 0) MIF_ST_F(&B[b2 + k * bp0], xb);
+      }
+    }
+  } else {
+    // X of each line as a real array overlaying its pair's buffer
+    T* xr = line(l);
+#pragma unroll
+    for (int q = 0; q < MQ; q++) {
+      const int m = t / NL + q * STEP;
+      if (m < N) xr[m] = val[q];
+    }
+    __syncthreads();
+    Cx<T> zk[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int it = t + 256 * r;
+      if (NI % 256 == 0 || it < NI) {
+        const int p = it % NP, k = it / NP;
+        const T* xa = line(p);
+        const T* xb = line(p + NP);
+        const T c = hc[k].re, sn = hc[k].im;
+        const T a0 = xa[k], a1 = k == 0 ? (T)0 : xa[N - k];
+        const T b0 = xb[k], b1 = k == 0 ? (T)0 : xb[N - k];
+        // V = (c + i s)(X_k - i X_{N-k}); Z = V^a + i V^b
+        const Cx<T> va = {c * a0 + sn * a1, sn * a0 - c * a1};
+        const Cx<T> vb = {c * b0 + sn * b1, sn * b0 - c * b1};
+        zk[r] = Cx<T>{va.re - vb.im, va.im + vb.re};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int it = t + 256 * r;
+      if (NI % 256 == 0 || it < NI) Z[it % NP][pd(it / NP)] = zk[r];
+    }
+    __syncthreads();
+    p2::fft<T, N, NP, +1>(Z, tws);
+    // x_j = v at the reordered position (N times the line)
+    if (b >= 0)
+      for (int j = t / NL; j < N; j += STEP) {
+        const int m = (j & 1) ? N - 1 - (j >> 1) : (j >> 1);
+        const Cx<T> z = Z[l % NP][pd(m)];
+        MIF_ST_F(&B[b + j * bp0], l < NP ? z.re : z.im);
+      }
+  }
+}
+
+template <typename T, int N, int NP>
+static void launch_dct2_y(const DGrid& g, const FftPlan& plan, int inverse, T* B, long bp0, long bplane,
+                          hipStream_t st) {
+  const T* W = sizeof(T) == 8 ? (const T*)plan.W64 : (const T*)plan.W32;
+  const T* H = sizeof(T) == 8 ? (const T*)plan.H64 : (const T*)plan.H32;
+  const dim3 grd((unsigned)((g.npts[0] + 2 * NP - 1) / (2 * NP)), (unsigned)g.npts[2], 1);
+  if (inverse)
+    k_dct2_y<T, N, NP, 1><<<grd, 256, 0, st>>>(g, W, H, B, bp0, bplane);
+  else
+    k_dct2_y<T, N, NP, 0><<<grd, 256, 0, st>>>(g, W, H, B, bp0, bplane);
+}
+
+// The cell-centred y pass; false for lengths without a kernel.
+template <typename T>
+bool launch_dct2_y_any(const DGrid& g, const FftPlan& plan, int inverse, T* B, long bp0, long bplane,
+                       hipStream_t st) {
+  constexpr bool d = sizeof(T) == 8;
+  switch (plan.L) {
+    case 64:
+      launch_dct2_y<T, 64, (d ? 8 : 16)>(g, plan, inverse, B, bp0, bplane, st);
+      return true;
+    case 128:
+      launch_dct2_y<T, 128, (d ? 8 : 8)>(g, plan, inverse, B, bp0, bplane, st);
+      return true;
+    case 256:
+      launch_dct2_y<T, 256, (d ? 4 : 8)>(g, plan, inverse, B, bp0, bplane, st);
+      return true;
+    case 512:
+      launch_dct2_y<T, 512, (d ? 2 : 4)>(g, plan, inverse, B, bp0, bplane, st);
+      return true;
+    case 1024:
+      launch_dct2_y<T, 1024, (d ? 1 : 2)>(g, plan, inverse, B, bp0, bplane, st);
+      return true;
+    default:
+      return false;
+  }
+}
+
+template bool launch_dct2_y_any<double>(const DGrid&, const FftPlan&, int, double*, long, long, hipStream_t);
+template bool launch_dct2_y_any<float>(const DGrid&, const FftPlan&, int, float*, long, long, hipStream_t);
+
 // Line pairs per 256-thread workgroup, per element type, length and pass:
 // 2*NP adjacent lines, i.e. runs of 2*NP*sizeof(T) contiguous bytes on the
 // strided (y, z) passes; the LDS footprint grows with NP*L, and fewer pairs
@@ -895,7 +1107,14 @@ static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, c
   const int per = plan.periodic;
 #define MIF_P2(AX, MD) k_fft_p2<T, L, NP, AX, MD, false><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, none)
   if constexpr (PASSES & 1) {
-    if (axis == 0 && io && sizeof(T) == 8 && io->fields_fp32) {
+    if (axis == 0 && io && io->dc) {
+      if constexpr (sizeof(T) == 4) {
+        if (mode == 0)
+          k_fft_p2<T, L, NP, 0, 0, true, T, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+        else
+          k_fft_p2<T, L, NP, 0, 1, true, T, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+      }
+    } else if (axis == 0 && io && sizeof(T) == 8 && io->fields_fp32) {
       if (mode == 0)
         k_fft_p2<T, L, NP, 0, 0, true, float><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
       else
@@ -996,6 +1215,26 @@ bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, co
     default:
       return false;
   }
+}
+
+// Workgroups of the x pass launch_fft_pow2 runs for a plan (its line pairs
+// per workgroup follow the same table); 0 when the length has no kernel.
+int fused_x_blocks(const DGrid& g, const FftPlan& plan, bool fp64) {
+  int np;
+  switch (plan.L) {
+    case 64: np = fp64 ? MIF_NP_D64 : MIF_NP_F64; break;
+    case 128: np = fp64 ? MIF_NP_D128 : MIF_NP_F128; break;
+    case 96: np = fp64 ? MIF_NP_D96 : MIF_NP_F96; break;
+    case 256: np = fp64 ? MIF_NPX_D256 : MIF_NPX_F256; break;
+    case 512: np = fp64 ? MIF_NPX_D512 : MIF_NPX_F512; break;
+    case 192: np = fp64 ? MIF_NPX_D192 : MIF_NPX_F192; break;
+    case 384: np = fp64 ? MIF_NPX_D384 : MIF_NPX_F384; break;
+    case 768: np = fp64 ? MIF_NPX_D768 : MIF_NPX_F768; break;
+    case 1024: case 1536: np = 1; break;
+    default: return 0;
+  }
+  const long lines = (long)g.npts[1] * g.npts[2];
+  return (int)((lines + 2 * np - 1) / (2 * np));
 }
 
 template bool launch_fft_pow2<double>(const DGrid&, int, int, const FftPlan&, const SpectralScale&, double*, long, long,

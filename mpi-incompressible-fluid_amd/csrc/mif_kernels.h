@@ -47,6 +47,11 @@ struct FftPlan {
   int Ns[24];
   const double* W64;
   const float* W32;
+  // cell-centred Neumann axis: DCT-II/DCT-III of length n = L, with the
+  // half-angle table (cos, sin)(pi k / (2L)), k < L
+  int dct2;
+  const double* H64;
+  const float* H32;
 };
 
 // Scaling applied between the forward and inverse transform of the last
@@ -68,6 +73,15 @@ struct FusedIO {
   // the velocity and pressure fields hold FP32 values while the transform
   // runs in FP64 (the mixed form with FP64 transforms)
   int fields_fp32 = 0;
+  // mixed form with one FP64 defect sweep (FP32 transforms): the forward
+  // pass also stores the FP64 divergence b (packed layout) and per-workgroup
+  // owner-weighted sums (W.b, W.1) into partial; the inverse pass adds the
+  // first FP32 solution add32 (packed layout) to the correction in FP64,
+  // stores the sum, and leaves (W.phi, W.1) of the stored values in partial
+  int dc = 0;
+  double* b64 = nullptr;
+  const float* add32 = nullptr;
+  Red3* partial = nullptr;
 };
 
 template <typename T>
@@ -93,7 +107,9 @@ void launch_pack(const DGrid& g, const Layout& L, const T* P, S* B, long bp0, lo
 template <typename T, typename S>
 void launch_unpack(const DGrid& g, const Layout& L, const S* B, long bp0, long bplane, T* P, hipStream_t st);
 template <typename T>
-void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st);
+// p += phi (minus mean->s1 / mean->s2 when mean is not null), velocity -= dts grad phi
+void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st,
+                    const Red3* mean = nullptr);
 template <typename T>
 void launch_pgrad(const DGrid& g, const Layout& L, const T* p, T* u, T* v, T* w, hipStream_t st);
 template <typename T>
@@ -137,6 +153,13 @@ bool fft_fast_len(int L);
 template <typename T>
 bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, const SpectralScale& sc, T* B, long bp0,
                      long bplane, const FusedIO* io, hipStream_t st);
+// Workgroups of the specialised x pass of a plan (0 without a kernel): the
+// number of partial sums a fused x pass with FusedIO::dc leaves.
+int fused_x_blocks(const DGrid& g, const FftPlan& plan, bool fp64);
+// DCT-II (forward) or DCT-III (inverse) pass of a cell-centred Neumann y
+// axis; false when the length (64, 128, 256, 512 or 1024) has no kernel.
+template <typename T>
+bool launch_dct2_y_any(const DGrid& g, const FftPlan& plan, int inverse, T* B, long bp0, long bplane, hipStream_t st);
 
 // Three component arrays of one velocity-shaped quantity.
 struct AdjVec {
@@ -175,6 +198,13 @@ void launch_remove_mean(const DGrid& g, double* P, long bp0, long bplane, Red3* 
 void launch_residual(const DGrid& g, const double* b, const double* P, float* B32, long bp0, long bplane,
                      hipStream_t st);
 void launch_accumulate(const DGrid& g, double* P, const float* B32, long bp0, long bplane, hipStream_t st);
+// One-sweep defect correction with FP32 transforms: R32 = (b - mean) - L phi_0
+// (FP64 arithmetic, phi_0 the FP32 first solution); the sum of n partial
+// (W.x, W.1) pairs into total; a node field minus the mean total holds.
+void launch_residual32(const DGrid& g, const double* b, const float* P, const Red3* total, float* R32, long bp0,
+                       long bplane, hipStream_t st);
+void launch_reduce_partials(const Red3* partial, int n, Red3* total, hipStream_t st);
+void launch_sub_mean_field(const DGrid& g, const Layout& L, float* F, const Red3* total, hipStream_t st);
 
 // Streaming-ceiling probe (mif_probe.hip): nr arrays read and nw arrays
 // written, n4 16-byte elements each.

@@ -1,5 +1,8 @@
 // mif_mixed.hip -- FP64 pieces of the mixed-precision pressure solve.
 //
+// On a cell-centred Neumann axis (stand-alone Poisson solves) the weights
+// are uniform and the ghost mirrors the first/last cell (phi_{-1} = phi_0).
+//
 // The mixed form stores and advances the velocity in FP32 and runs the
 // spectral transforms in FP32.  What stays FP64 (no counterpart in the
 // reference, which is uniformly FP64 or uniformly FP32, include/Real.h:9-17):
@@ -17,10 +20,7 @@
 
 namespace mif {
 
-__device__ inline double axis_weight(const DGrid& g, int a, int i) {
-  if (g.periodic[a]) return 1.0;
-  return (i == 0 || i == g.npts[a] - 1) ? 0.5 : 1.0;
-}
+__device__ inline double axis_weight(const DGrid& g, int a, int i) { return owner_weight(g, a, i); }
 
 // Partial sums of (W.b, W.1) over the packed buffer.
 __global__ void __launch_bounds__(256) k_wsum(DGrid g, const double* __restrict__ B, long bp0, long bplane,
@@ -83,6 +83,10 @@ __device__ inline long nb(const DGrid& g, int a, int i, int d) {
   int q = i + d;
   if (g.periodic[a]) {
     q = q < 0 ? q + n : (q >= n ? q - n : q);
+  } else if (g.cell[a]) {
+    // cell-centred: the ghost mirrors the first/last cell
+    if (q < 0) q = 0;
+    if (q >= n) q = n - 1;
   } else {
     if (q < 0) q = n > 1 ? 1 : 0;
     if (q >= n) q = n > 1 ? n - 2 : 0;
@@ -105,6 +109,40 @@ __global__ void __launch_bounds__(256) k_residual(DGrid g, const double* __restr
   const double lz = (P[i + bp0 * j + bplane * nb(g, 2, k, -1)] - 2 * c + P[i + bp0 * j + bplane * nb(g, 2, k, 1)]) /
                     (g.h[2] * g.h[2]);
   B32[o] = (float)(b[o] - (lx + ly + lz));
+}
+
+// One-sweep defect correction (FP32 transforms): R32 = (float)((b - mean_b)
+// - L phi_0) in FP64, phi_0 read from the FP32 first solution; mean_b =
+// total->s1 / total->s2, the owner-weighted mean of b (the projection of the
+// right-hand side onto the range of the operator).
+__global__ void __launch_bounds__(256) k_residual32(DGrid g, const double* __restrict__ b, const float* __restrict__ P,
+                                                    const Red3* __restrict__ total, float* __restrict__ R32, long bp0,
+                                                    long bplane) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.npts[0]) return;
+  const long o = i + bp0 * j + bplane * k;
+  const double mean = total->s1 / total->s2;
+  const double c = P[o];
+  const double lx = ((double)P[nb(g, 0, i, -1) + bp0 * j + bplane * k] - 2 * c +
+                     (double)P[nb(g, 0, i, 1) + bp0 * j + bplane * k]) /
+                    (g.h[0] * g.h[0]);
+  const double ly = ((double)P[i + bp0 * nb(g, 1, j, -1) + bplane * k] - 2 * c +
+                     (double)P[i + bp0 * nb(g, 1, j, 1) + bplane * k]) /
+                    (g.h[1] * g.h[1]);
+  const double lz = ((double)P[i + bp0 * j + bplane * nb(g, 2, k, -1)] - 2 * c +
+                     (double)P[i + bp0 * j + bplane * nb(g, 2, k, 1)]) /
+                    (g.h[2] * g.h[2]);
+  R32[o] = (float)((b[o] - mean) - (lx + ly + lz));
+}
+
+// Subtract total->s1 / total->s2 from every point of a node field.
+__global__ void __launch_bounds__(256) k_sub_mean_field(DGrid g, Layout L, float* __restrict__ F,
+                                                        const Red3* __restrict__ total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.N[0]) return;
+  F[L.idx(i, j, k)] -= (float)(total->s1 / total->s2);
 }
 
 __global__ void __launch_bounds__(256) k_accumulate(DGrid g, double* __restrict__ P, const float* __restrict__ B32,
@@ -163,6 +201,21 @@ void launch_residual(const DGrid& g, const double* b, const double* P, float* B3
                      hipStream_t st) {
   dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
   k_residual<<<grd, 256, 0, st>>>(g, b, P, B32, bp0, bplane);
+}
+
+void launch_residual32(const DGrid& g, const double* b, const float* P, const Red3* total, float* R32, long bp0,
+                       long bplane, hipStream_t st) {
+  dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
+  k_residual32<<<grd, 256, 0, st>>>(g, b, P, total, R32, bp0, bplane);
+}
+
+void launch_reduce_partials(const Red3* partial, int n, Red3* total, hipStream_t st) {
+  k_reduce_sum2<<<1, 256, 0, st>>>(partial, n, total);
+}
+
+void launch_sub_mean_field(const DGrid& g, const Layout& L, float* F, const Red3* total, hipStream_t st) {
+  dim3 grd((g.N[0] + 255) / 256, g.N[1], g.N[2]);
+  k_sub_mean_field<<<grd, 256, 0, st>>>(g, L, F, total);
 }
 
 void launch_accumulate(const DGrid& g, double* P, const float* B32, long bp0, long bplane, hipStream_t st) {

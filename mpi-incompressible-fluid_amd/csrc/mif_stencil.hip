@@ -93,9 +93,7 @@ __device__ inline bool interior(const DGrid& g, int c, int i, int j, int k) {
 // 3.10 -> 2.99 ms, mixed 1.636 -> 1.565 ms).  Loads that neighbouring points
 // or tiles read again stay ordinary: non-temporal halo-plane loads took the
 // mixed stage kernel from 180 to 196 us.
-#ifndef MIF_NT_LOADS
-#define MIF_NT_LOADS 1
-#endif
+constexpr bool kNtLoads = true;
 template <bool NT, typename T>
 __device__ inline T load_in(const T* addr) {
   if constexpr (NT)
@@ -114,22 +112,10 @@ __device__ inline void store_out(T* addr, T val) {
 // per element type.  Measured at 256^3 against 64 x 8: 128 x 4 took the FP32
 // stage kernel from 182.5 to 177.7 us (mixed step 1.562 -> 1.552 ms) and the
 // FP64 one from 335 to 345 us.
-#ifndef MIF_STX32
-#define MIF_STX32 128
-#endif
-#ifndef MIF_STY32
-#define MIF_STY32 4
-#endif
-#ifndef MIF_STX64
-#define MIF_STX64 64
-#endif
-#ifndef MIF_STY64
-#define MIF_STY64 8
-#endif
 template <typename T>
 struct StageTile {
-  static constexpr int STX = sizeof(T) == 4 ? MIF_STX32 : MIF_STX64;
-  static constexpr int STY = sizeof(T) == 4 ? MIF_STY32 : MIF_STY64;
+  static constexpr int STX = sizeof(T) == 4 ? 128 : 64;
+  static constexpr int STY = sizeof(T) == 4 ? 4 : 8;
   static constexpr int SHX = STX + 2, SHY = STY + 2;
   static constexpr int STN = STX * STY;                      // threads per workgroup
   static constexpr int PLANE_ELEMS = 3 * SHY * SHX;          // one halo plane of u, v, w
@@ -140,18 +126,11 @@ struct StageTile {
 // planes 180.4 us, 16 176.5 us, 24 178.0 us, 32 192.5 us)
 constexpr int SKB = 16;
 
-#ifndef MIF_STAGE_NS32
-#define MIF_STAGE_NS32 4
-#endif
-#ifndef MIF_STAGE_VEC
-#define MIF_STAGE_VEC 1
-#endif
-#ifndef MIF_STAGE_PCARRY
-#define MIF_STAGE_PCARRY 1
-#endif
-#ifndef MIF_STAGE_NS64
-#define MIF_STAGE_NS64 3
-#endif
+// ring slots of the FP32 and FP64 stage kernels (see k_stage)
+constexpr int kStageSlots32 = 4, kStageSlots64 = 3;
+// halo planes fetched with 16-byte loads where the storage shift aligns the
+// rows (see the launcher)
+constexpr bool kStageVec = true;
 
 // FT (velocity-only step): MIF_FN_VT_FORCING read from the trig tables.
 // VEC: halo planes fetched with 16-byte loads (element 1 of every row
@@ -162,12 +141,12 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 and
   // more (plane k+2 is committed into a free slot while plane k is
   // computed; one barrier per plane)
-  constexpr int NS = sizeof(T) == 4 ? MIF_STAGE_NS32 : MIF_STAGE_NS64;
+  constexpr int NS = sizeof(T) == 4 ? kStageSlots32 : kStageSlots64;
   // p at plane k-1 carried in a register from plane to plane instead of
   // loaded (FP64: 256^3 stage kernel 348 -> 341 us, step 2.955 -> 2.928 ms;
   // FP32: 181 -> 183 us on 64 x 8 tiles, 176 -> 179 us on 128 x 4, so FP32
   // keeps the load)
-  constexpr bool PCARRY = MIF_STAGE_PCARRY && sizeof(T) == 8;
+  constexpr bool PCARRY = sizeof(T) == 8;
   using TT = StageTile<T>;
   constexpr int STX = TT::STX, STY = TT::STY, SHX = TT::SHX, SHY = TT::SHY, STN = TT::STN;
   constexpr int PLANE_ELEMS = TT::PLANE_ELEMS, PQ = TT::PQ;
@@ -312,7 +291,7 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
         if constexpr (KIND >= 2) {
 #pragma unroll
           for (int c = 0; c < 3; c++)
-            if (inside[c]) av[c] = load_in<NT && MIF_NT_LOADS>(&((const T*)s.A[c])[o]);
+            if (inside[c]) av[c] = load_in<NT && kNtLoads>(&((const T*)s.A[c])[o]);
         }
       } else if constexpr (KIND >= 5) {
 #pragma unroll
@@ -428,12 +407,16 @@ struct BCArgs {
   int count[3][3];  // threads (opposite shell points) per (component, face pair)
 };
 
-// ZERO: homogeneous data (no-slip walls), every face value is 0; the
-// function evaluation code is compiled out, which keeps the kernel small.
+// Face data of a boundary pass: BC_ZERO homogeneous data (no-slip walls),
+// every face value is 0 and the function evaluation code is compiled out,
+// which keeps the kernel small; BC_FN closed-form descriptors evaluated on
+// the device; BC_TABLE values evaluated on the host (FnDesc kFnFaceTable).
+enum BCMode { BC_ZERO = 0, BC_FN = 1, BC_TABLE = 2 };
+
 // Value of shell point (i, j, k) of component c: a Dirichlet face value, or
 // the value at its periodic source, which is then an interior point that
 // this pass does not write.
-template <typename T, bool ZERO>
+template <typename T, int MODE>
 __device__ inline T shell_value(const DGrid& g, const Layout& L, const BCArgs& a, int c, const int e[3], int i, int j,
                                 int k, const T* F) {
   // periodic ghost source
@@ -460,8 +443,10 @@ __device__ inline T shell_value(const DGrid& g, const Layout& L, const BCArgs& a
     if (s[0] == e[0] - 1) face = 5;
   }
   if (face >= 0) {
-    if constexpr (ZERO)
+    if constexpr (MODE == BC_ZERO)
       return (T)0;
+    else if constexpr (MODE == BC_TABLE)
+      return (T)a.f.tab[a.f.toff[c][face] + face_plane_index(face, e, s[0], s[1], s[2])];
     else
       return (T)face_value(g, a.f, a.t, c, face, s[0], s[1], s[2]);
   }
@@ -474,7 +459,7 @@ __device__ inline T shell_value(const DGrid& g, const Layout& L, const BCArgs& a
 // two points is the other's source-side neighbour (ghost 0 and point 1 share
 // a row segment, as do e-2 and ghost e-1), so both reads come first and the
 // two writes then land on lines the same thread has just read.
-template <typename T, bool ZERO>
+template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
   const int c = blockIdx.y / 3, pair = blockIdx.y % 3;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,8 +486,8 @@ __global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
   int p1[3] = {i, j, k};
   p1[axis] = e[axis] - 1;
   T* F = (T*)a.F[c];
-  const T v0 = shell_value<T, ZERO>(g, L, a, c, e, i, j, k, F);
-  const T v1 = shell_value<T, ZERO>(g, L, a, c, e, p1[0], p1[1], p1[2], F);
+  const T v0 = shell_value<T, MODE>(g, L, a, c, e, i, j, k, F);
+  const T v1 = shell_value<T, MODE>(g, L, a, c, e, p1[0], p1[1], p1[2], F);
   F[L.idx(i, j, k)] = v0;
   F[L.idx(p1[0], p1[1], p1[2])] = v1;
 }
@@ -578,7 +563,7 @@ __global__ void __launch_bounds__(256) k_set(DGrid g, Layout L, T* F, int stag, 
 // (i,j,k) -> (i-olo0, j-olo1, k-olo2), PressureTensor::copy_from_staggered,
 // src/PressureTensor.cpp:16-19).
 // ---------------------------------------------------------------------------
-template <typename T, typename S, bool NHN>
+template <typename T, typename S, int NHN>
 __global__ void __launch_bounds__(256) k_div_rhs(DGrid g, Layout L, const T* __restrict__ u, const T* __restrict__ v,
                                                  const T* __restrict__ w, S* __restrict__ B, long bp0, long bplane,
                                                  double dt, FnDesc gp, double tg) {
@@ -592,7 +577,7 @@ __global__ void __launch_bounds__(256) k_div_rhs(DGrid g, Layout L, const T* __r
   const double dv_dy = ((double)v[o + L.PX] - (double)v[o]) * g.one_over_h[1];
   const double dw_dz = ((double)w[o + L.plane] - (double)w[o]) * g.one_over_h[2];
   double r = (du_dx + dv_dy + dw_dz) / dt;
-  if (NHN) {
+  if (NHN == 1) {
     const int sx = g.N[0], sy = g.N[1], sz = g.N[2];
     if (k == 0) r += eval_at(g, gp, 3, 2, tg, i, j, k) * 2.0 * g.one_over_h[2];
     if (k == sz - 1) r -= eval_at(g, gp, 3, 2, tg, i, j, k) * 2.0 * g.one_over_h[2];
@@ -600,6 +585,16 @@ __global__ void __launch_bounds__(256) k_div_rhs(DGrid g, Layout L, const T* __r
     if (j == sy - 1) r -= eval_at(g, gp, 3, 1, tg, i, j, k) * 2.0 * g.one_over_h[1];
     if (i == 0) r += eval_at(g, gp, 3, 0, tg, i, j, k) * 2.0 * g.one_over_h[0];
     if (i == sx - 1) r -= eval_at(g, gp, 3, 0, tg, i, j, k) * 2.0 * g.one_over_h[0];
+  } else if (NHN == 2) {
+    // host-evaluated normal gradient components at the wall nodes
+    const int e[3] = {g.N[0], g.N[1], g.N[2]};
+    const double* t = gp.tab;
+    if (k == 0) r += t[gp.toff[0][0] + face_plane_index(0, e, i, j, k)] * 2.0 * g.one_over_h[2];
+    if (k == e[2] - 1) r -= t[gp.toff[0][1] + face_plane_index(1, e, i, j, k)] * 2.0 * g.one_over_h[2];
+    if (j == 0) r += t[gp.toff[0][2] + face_plane_index(2, e, i, j, k)] * 2.0 * g.one_over_h[1];
+    if (j == e[1] - 1) r -= t[gp.toff[0][3] + face_plane_index(3, e, i, j, k)] * 2.0 * g.one_over_h[1];
+    if (i == 0) r += t[gp.toff[0][4] + face_plane_index(4, e, i, j, k)] * 2.0 * g.one_over_h[0];
+    if (i == e[0] - 1) r -= t[gp.toff[0][5] + face_plane_index(5, e, i, j, k)] * 2.0 * g.one_over_h[0];
   }
   B[a + bp0 * b + bplane * c] = (S)r;
 }
@@ -646,7 +641,7 @@ __global__ void __launch_bounds__(256) k_unpack(DGrid g, Layout L, const S* __re
 template <typename T, bool NT>
 __global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restrict__ u, T* __restrict__ v,
                                                  T* __restrict__ w, T* __restrict__ p, const T* __restrict__ phi,
-                                                 double dts, int nqt, int kchunk) {
+                                                 double dts, int nqt, int kchunk, const Red3* __restrict__ mean) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int qt = tile % nqt, kt = tile / nqt;
   // slices start at the allocation's aligned origin, L.lead elements
@@ -664,12 +659,15 @@ __global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restric
     d.f = phi[o];
     d.fx = phi[o >= 1 ? o - 1 : o];
     d.fy = phi[o >= L.PX ? o - L.PX : o];
-    d.pp = load_in<NT && MIF_NT_LOADS>(&p[o]);
-    d.a[0] = load_in<NT && MIF_NT_LOADS>(&u[o]);
-    d.a[1] = load_in<NT && MIF_NT_LOADS>(&v[o]);
-    d.a[2] = load_in<NT && MIF_NT_LOADS>(&w[o]);
+    d.pp = load_in<NT && kNtLoads>(&p[o]);
+    d.a[0] = load_in<NT && kNtLoads>(&u[o]);
+    d.a[1] = load_in<NT && kNtLoads>(&v[o]);
+    d.a[2] = load_in<NT && kNtLoads>(&w[o]);
   };
   T fz = k0 >= 1 ? phi[(long)q + L.plane * (k0 - 1)] : (T)0;
+  // the increment's weighted mean, when the solve left it in phi (the mixed
+  // form's one-sweep defect correction): p += phi - mean
+  const T pm = mean ? (T)(mean->s1 / mean->s2) : (T)0;
   Pt cur;
   fetch(k0, cur);
   T* F[3] = {u, v, w};
@@ -677,7 +675,7 @@ __global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restric
     Pt nxt = cur;
     if (k + 1 < k1) fetch(k + 1, nxt);
     const long o = (long)q + L.plane * k;
-    if (i < g.N[0] && j < g.N[1] && k < g.N[2]) store_out<NT>(&p[o], cur.pp + cur.f);
+    if (i < g.N[0] && j < g.N[1] && k < g.N[2]) store_out<NT>(&p[o], cur.pp + (cur.f - pm));
     const T d[3] = {cur.f - cur.fx, cur.f - cur.fy, cur.f - fz};
 #pragma unroll
     for (int c = 0; c < 3; c++)
@@ -884,7 +882,7 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   // with the LDS ring padded to six slots, i.e. three workgroups per CU as
   // with scalar loads), so FP32 with non-temporal stores keeps the scalar
   // loads (on the FP32 128 x 4 tiles: 176.2 -> 175.9 us, no gain either).
-  if constexpr (MIF_STAGE_VEC && KIND <= 3 && (sizeof(T) == 8 || !NT)) {
+  if constexpr (kStageVec && KIND <= 3 && (sizeof(T) == 8 || !NT)) {
     if (L.lead * (long)sizeof(T) == 128 - (long)sizeof(T) && (L.PX * sizeof(T)) % 16 == 0) {
       k_stage<T, KIND, FT, NT, true>
           <<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
@@ -955,9 +953,11 @@ void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& 
   a.t = t;
   dim3 grd((mx + 255) / 256, 9, 1);
   if (f.kind == MIF_FN_ZERO)
-    k_bc<T, true><<<grd, 256, 0, st>>>(g, L, a);
+    k_bc<T, BC_ZERO><<<grd, 256, 0, st>>>(g, L, a);
+  else if (f.kind == kFnFaceTable)
+    k_bc<T, BC_TABLE><<<grd, 256, 0, st>>>(g, L, a);
   else
-    k_bc<T, false><<<grd, 256, 0, st>>>(g, L, a);
+    k_bc<T, BC_FN><<<grd, 256, 0, st>>>(g, L, a);
 }
 
 template <typename T>
@@ -983,10 +983,12 @@ template <typename T, typename S>
 void launch_div_rhs(const DGrid& g, const Layout& L, const T* u, const T* v, const T* w, S* B, long bp0, long bplane,
                     double dt, const FnDesc& gp, double tg, int nhn, hipStream_t st) {
   dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
-  if (nhn)
-    k_div_rhs<T, S, true><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg);
+  if (nhn && gp.kind == kFnFaceTable)
+    k_div_rhs<T, S, 2><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg);
+  else if (nhn)
+    k_div_rhs<T, S, 1><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg);
   else
-    k_div_rhs<T, S, false><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg);
+    k_div_rhs<T, S, 0><<<grd, 256, 0, st>>>(g, L, u, v, w, B, bp0, bplane, dt, gp, tg);
 }
 
 template <typename T, typename S>
@@ -1002,7 +1004,8 @@ void launch_unpack(const DGrid& g, const Layout& L, const S* B, long bp0, long b
 }
 
 template <typename T>
-void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st) {
+void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st,
+                    const Red3* mean) {
   // points 0..N per axis (the union of the extents; u's x+ ghost included,
   // the interior tests filter), as 256-position slices of the padded plane
   // times chunks of 16 planes
@@ -1010,9 +1013,9 @@ void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, con
   const int nqt = (int)((L.plane + L.lead + 255) / 256);
   const int nkt = (g.N[2] + 1 + kchunk - 1) / kchunk;
   if (L.nt)
-    k_project<T, true><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk);
+    k_project<T, true><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk, mean);
   else
-    k_project<T, false><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk);
+    k_project<T, false><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk, mean);
 }
 
 template <typename T>
@@ -1088,7 +1091,8 @@ void launch_gather(const Layout& L, const T* F, const int* ijk, long n, double* 
   template void launch_bc<T>(const DGrid&, const Layout&, void* const*, const FnDesc&, double, hipStream_t);         \
   template void launch_periodic<T>(const DGrid&, const Layout&, T*, int, hipStream_t);                              \
   template void launch_set<T>(const DGrid&, const Layout&, T*, int, int, const FnDesc&, double, int, hipStream_t);  \
-  template void launch_project<T>(const DGrid&, const Layout&, T*, T*, T*, T*, const T*, double, hipStream_t);      \
+  template void launch_project<T>(const DGrid&, const Layout&, T*, T*, T*, T*, const T*, double, hipStream_t,       \
+                                  const Red3*);                                                                     \
   template void launch_pgrad<T>(const DGrid&, const Layout&, const T*, T*, T*, T*, hipStream_t);                    \
   template void launch_from_dense<T>(const Layout&, const int*, const double*, T*, hipStream_t);                    \
   template void launch_to_dense<T>(const Layout&, const int*, const T*, double*, hipStream_t);                      \

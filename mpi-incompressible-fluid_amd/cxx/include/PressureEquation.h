@@ -11,8 +11,8 @@ namespace mif {
 void solve_pressure_equation_homogeneous_periodic(StaggeredTensor &pressure, PressureTensor &pressure_buffer,
                                                   const VelocityTensor &velocity, Real dt);
 
-// exact_pressure_gradient must carry a descriptor (it is evaluated on the
-// boundary planes on the device).
+// exact_pressure_gradient is evaluated on the boundary planes on the device
+// when it carries a descriptor, else on the host into a Neumann face table.
 void solve_pressure_equation_non_homogeneous_neumann(StaggeredTensor &pressure, PressureTensor &pressure_buffer,
                                                      const VelocityTensor &velocity,
                                                      const VectorFunction &exact_pressure_gradient, Real dt);

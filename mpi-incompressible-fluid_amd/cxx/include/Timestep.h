@@ -9,8 +9,9 @@
 namespace mif {
 
 // Homogeneous Neumann / periodic pressure increment (src/Timestep.cpp:
-// 97-141).  exact_velocity gives the Dirichlet data and must carry a
-// descriptor.
+// 97-141).  exact_velocity gives the Dirichlet data: evaluated on the
+// device when it carries a descriptor, else on the host at the three stage
+// times into face tables (O(N^2) callback evaluations per stage).
 void timestep(VelocityTensor &velocity, VelocityTensor &velocity_buffer, VelocityTensor &velocity_buffer_2,
               const TimeVectorFunction &exact_velocity, Real t_n, StaggeredTensor &pressure,
               StaggeredTensor &pressure_buffer, PressureTensor &pressure_solver_buffer);

@@ -6,11 +6,11 @@
 // built from one of the reference's named free functions (u_exact, ...,
 // forcing_x, ..., exact_u_t1, ...) is recognised and carries the matching
 // mif_function descriptor; set_time and get_difference_over_time keep it.
-// A descriptor can also be given directly.  Routines that evaluate boundary
-// data on the device every stage (timestep, apply_bc, the Neumann data of
-// the pressure solve) need the descriptor and throw std::invalid_argument
-// without one; set, adjust_pressure and the error norms fall back to
-// evaluating the callbacks on the host.
+// A descriptor can also be given directly.  Without one, the callbacks are
+// evaluated on the host: boundary data (timestep, timestep_nhn,
+// timestep_velocity, apply_bc, the Neumann data of the pressure solve) into
+// O(N^2) face tables per stage, and set, adjust_pressure and the error norms
+// into whole fields.
 #ifndef MIF_SHIM_VECTOR_FUNCTION_H
 #define MIF_SHIM_VECTOR_FUNCTION_H
 

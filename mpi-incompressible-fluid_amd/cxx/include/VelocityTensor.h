@@ -76,8 +76,8 @@ class VelocityTensor {
   void set(const VectorFunction &f, bool include_border);
 
   // Dirichlet data on the non-periodic faces and periodic copies
-  // (src/VelocityTensor.cpp:36-233), on the device; f must carry a
-  // descriptor.
+  // (src/VelocityTensor.cpp:36-233), on the device; f is evaluated there
+  // when it carries a descriptor, else on the host into face tables.
   void apply_bc(const VectorFunction &exact_velocity);
 
   // The three device fields, with newer host values uploaded first.

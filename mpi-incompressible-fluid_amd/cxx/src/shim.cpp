@@ -300,20 +300,107 @@ void VelocityTensor::set(const VectorFunction &f, bool include_border) {
 
 namespace {
 
-mif_function require_descriptor(bool has, const mif_function &d, const char *what) {
-  if (!has)
-    throw std::invalid_argument(std::string(what) +
-                                ": the device evaluates this function at every stage, so it must be one of the "
-                                "named exact solutions or carry a mif_function descriptor");
-  return d;
+// ---- boundary data of callbacks without a descriptor ----------------------
+// The device evaluates the named exact solutions itself.  Any other callback
+// is evaluated here, on the host, into the face tables of the C ABI's
+// *_tabulated entry points (layout in mif_gpu.h): O(N^2) values per stage.
+
+// The two axes other than `axis`, lower first.
+void other_axes(int axis, int *a, int *b) {
+  *a = axis == 0 ? 1 : 0;
+  *b = axis == 2 ? 1 : 2;
+}
+
+// Appends one velocity face table: the values VelocityTensor::apply_bc
+// writes (src/VelocityTensor.cpp:36-233).  On a face of axis `axis` the
+// tangential components take f at their own points of that face, at index 0
+// or extent - 1; the normal component takes f at the wall node (index 0, or
+// the last node N - 1) plus or minus h/2 times the tangential divergence of
+// f taken at the components' staggered points of that node.
+void append_velocity_faces(const VelocityTensor &vel, const VectorFunction &f, std::vector<double> &out) {
+  const Constants &c = vel.constants;
+  const size_t nodes[3] = {c.Nx, c.Ny, c.Nz};
+  const Real half[3] = {c.dx_over_2, c.dy_over_2, c.dz_over_2};
+  const Real inv_h[3] = {c.one_over_dx, c.one_over_dy, c.one_over_dz};
+  for (int q = 0; q < 3; q++) {
+    const StaggeredTensor &t = *vel.components[q];
+    const auto &e = t.sizes();
+    const SpaceFunction &fq = *f.components[q];
+    for (int face = 0; face < 6; face++) {
+      const int axis = 2 - face / 2;
+      const bool hi = face & 1;
+      if (c.periodic_bc[axis]) continue;
+      int o1, o2;
+      other_axes(axis, &o1, &o2);
+      for (size_t b = 0; b < e[o2]; b++)
+        for (size_t a = 0; a < e[o1]; a++) {
+          int idx[3];
+          idx[o1] = static_cast<int>(a);
+          idx[o2] = static_cast<int>(b);
+          if (q != axis) {
+            idx[axis] = hi ? static_cast<int>(e[axis]) - 1 : 0;
+            out.push_back(t.evaluate_function_at_index(idx[0], idx[1], idx[2], fq));
+            continue;
+          }
+          idx[axis] = hi ? static_cast<int>(nodes[axis]) - 1 : 0;
+          const Real at_boundary = t.evaluate_function_at_index_unstaggered(idx[0], idx[1], idx[2], fq);
+          Real div = 0;
+          for (const int o : {o1, o2}) {
+            const StaggeredTensor &s = *vel.components[o];
+            int nxt[3] = {idx[0], idx[1], idx[2]};
+            nxt[o] += 1;
+            const Real d = (s.evaluate_function_at_index(nxt[0], nxt[1], nxt[2], *f.components[o]) -
+                            s.evaluate_function_at_index(idx[0], idx[1], idx[2], *f.components[o])) *
+                           inv_h[o];
+            div = o == o1 ? d : div + d;
+          }
+          out.push_back(hi ? at_boundary - half[axis] * div : at_boundary + half[axis] * div);
+        }
+    }
+  }
+}
+
+// Appends one Neumann table: the normal component of g at every node of the
+// six wall faces of the node extents (apply_non_homogeneous_neumann,
+// src/PressureEquation.cpp:10-57, reads g there).
+void append_neumann_faces(const StaggeredTensor &pressure, const VectorFunction &g, std::vector<double> &out) {
+  const auto &e = pressure.sizes();
+  for (int face = 0; face < 6; face++) {
+    const int axis = 2 - face / 2;
+    const bool hi = face & 1;
+    int o1, o2;
+    other_axes(axis, &o1, &o2);
+    for (size_t b = 0; b < e[o2]; b++)
+      for (size_t a = 0; a < e[o1]; a++) {
+        int idx[3];
+        idx[o1] = static_cast<int>(a);
+        idx[o2] = static_cast<int>(b);
+        idx[axis] = hi ? static_cast<int>(e[axis]) - 1 : 0;
+        out.push_back(pressure.evaluate_function_at_index(idx[0], idx[1], idx[2], *g.components[axis]));
+      }
+  }
+}
+
+// The three stage times of FULL_TIMESTEP (src/Timestep.cpp:99-104): t_n +
+// 64/120 dt, then + 16/120 dt, then t_n + dt; index 0 is t_n.
+std::array<Real, 4> full_step_times(Real dt, Real t_n) {
+  const Real time_1 = t_n + 64.0 / 120.0 * dt;
+  const Real time_2 = time_1 + 16.0 / 120.0 * dt;
+  return {t_n, time_1, time_2, t_n + dt};
 }
 
 }  // namespace
 
 void VelocityTensor::apply_bc(const VectorFunction &exact_velocity) {
-  const mif_function d = require_descriptor(exact_velocity.has_descriptor(), exact_velocity.descriptor(), "apply_bc");
   auto f = device();
-  check_status(mif_velocity_apply_bc(f.data(), &d, exact_velocity.time()), constants.context(), "apply_bc");
+  if (exact_velocity.has_descriptor()) {
+    const mif_function d = exact_velocity.descriptor();
+    check_status(mif_velocity_apply_bc(f.data(), &d, exact_velocity.time()), constants.context(), "apply_bc");
+  } else {
+    std::vector<double> faces;
+    append_velocity_faces(*this, exact_velocity, faces);
+    check_status(mif_velocity_apply_bc_tabulated(f.data(), faces.data()), constants.context(), "apply_bc");
+  }
   mark_device_written();
 }
 
@@ -353,10 +440,19 @@ void PressureTensor::copy_to_staggered(StaggeredTensor &other, int) const {
 void timestep(VelocityTensor &velocity, VelocityTensor &velocity_buffer, VelocityTensor &velocity_buffer_2,
               const TimeVectorFunction &exact_velocity, Real t_n, StaggeredTensor &pressure,
               StaggeredTensor &pressure_buffer, PressureTensor &) {
-  const mif_function ev = require_descriptor(exact_velocity.has_descriptor(), exact_velocity.descriptor(), "timestep");
   auto a = velocity.device(), b = velocity_buffer.device(), c = velocity_buffer_2.device();
-  check_status(mif_timestep(a.data(), b.data(), c.data(), &ev, t_n, pressure.device(), pressure_buffer.device()),
-               velocity.constants.context(), "timestep");
+  if (exact_velocity.has_descriptor()) {
+    const mif_function ev = exact_velocity.descriptor();
+    check_status(mif_timestep(a.data(), b.data(), c.data(), &ev, t_n, pressure.device(), pressure_buffer.device()),
+                 velocity.constants.context(), "timestep");
+  } else {
+    const auto times = full_step_times(velocity.constants.dt, t_n);
+    std::vector<double> faces;
+    for (int s = 1; s <= 3; s++) append_velocity_faces(velocity, exact_velocity.set_time(times[s]), faces);
+    check_status(
+        mif_timestep_tabulated(a.data(), b.data(), c.data(), faces.data(), t_n, pressure.device(), pressure_buffer.device()),
+        velocity.constants.context(), "timestep");
+  }
   velocity.mark_device_written();
   velocity_buffer.mark_device_written();
   velocity_buffer_2.mark_device_written();
@@ -367,14 +463,28 @@ void timestep(VelocityTensor &velocity, VelocityTensor &velocity_buffer, Velocit
 void timestep_nhn(VelocityTensor &velocity, VelocityTensor &velocity_buffer, VelocityTensor &velocity_buffer_2,
                   const TimeVectorFunction &exact_velocity, const TimeVectorFunction &exact_pressure_gradient,
                   Real t_n, StaggeredTensor &pressure, StaggeredTensor &pressure_buffer, PressureTensor &) {
-  const mif_function ev =
-      require_descriptor(exact_velocity.has_descriptor(), exact_velocity.descriptor(), "timestep_nhn");
-  const mif_function gp = require_descriptor(exact_pressure_gradient.has_descriptor(),
-                                             exact_pressure_gradient.descriptor(), "timestep_nhn");
   auto a = velocity.device(), b = velocity_buffer.device(), c = velocity_buffer_2.device();
-  check_status(
-      mif_timestep_nhn(a.data(), b.data(), c.data(), &ev, &gp, t_n, pressure.device(), pressure_buffer.device()),
-      velocity.constants.context(), "timestep_nhn");
+  if (exact_velocity.has_descriptor() && exact_pressure_gradient.has_descriptor()) {
+    const mif_function ev = exact_velocity.descriptor();
+    const mif_function gp = exact_pressure_gradient.descriptor();
+    check_status(
+        mif_timestep_nhn(a.data(), b.data(), c.data(), &ev, &gp, t_n, pressure.device(), pressure_buffer.device()),
+        velocity.constants.context(), "timestep_nhn");
+  } else {
+    // PRESSURE_EQUATION_true hands stage s's solve
+    // exact_pressure_gradient.get_difference_over_time(new_time, prev_time)
+    // (src/Timestep.cpp:89-95)
+    const auto times = full_step_times(velocity.constants.dt, t_n);
+    std::vector<double> faces, neumann;
+    for (int s = 1; s <= 3; s++) {
+      append_velocity_faces(velocity, exact_velocity.set_time(times[s]), faces);
+      append_neumann_faces(pressure, exact_pressure_gradient.get_difference_over_time(times[s], times[s - 1]),
+                           neumann);
+    }
+    check_status(mif_timestep_nhn_tabulated(a.data(), b.data(), c.data(), faces.data(), neumann.data(), t_n,
+                                            pressure.device(), pressure_buffer.device()),
+                 velocity.constants.context(), "timestep_nhn");
+  }
   velocity.mark_device_written();
   velocity_buffer.mark_device_written();
   velocity_buffer_2.mark_device_written();
@@ -384,14 +494,24 @@ void timestep_nhn(VelocityTensor &velocity, VelocityTensor &velocity_buffer, Vel
 
 void timestep_velocity(VelocityTensor &velocity, VelocityTensor &velocity_buffer, VelocityTensor &rhs_buffer,
                        const TimeVectorFunction &exact_velocity, Real t_n) {
-  const mif_function ev =
-      require_descriptor(exact_velocity.has_descriptor(), exact_velocity.descriptor(), "timestep_velocity");
   mif_function forcing{};
   forcing.kind = MIF_FN_VT_FORCING;
   forcing.Re = current_reynolds();
   auto a = velocity.device(), b = velocity_buffer.device(), c = rhs_buffer.device();
-  check_status(mif_timestep_velocity(a.data(), b.data(), c.data(), &ev, &forcing, t_n),
-               velocity.constants.context(), "timestep_velocity");
+  if (exact_velocity.has_descriptor()) {
+    const mif_function ev = exact_velocity.descriptor();
+    check_status(mif_timestep_velocity(a.data(), b.data(), c.data(), &ev, &forcing, t_n),
+                 velocity.constants.context(), "timestep_velocity");
+  } else {
+    // boundary data at t_n + 8/15 dt, t_n + 2/3 dt, t_n + dt
+    // (src/TimestepVelocity.cpp:60-90)
+    const Real dt = velocity.constants.dt;
+    const Real times[3] = {t_n + (8.0 / 15.0) * dt, t_n + (2.0 / 3.0) * dt, t_n + dt};
+    std::vector<double> faces;
+    for (const Real t : times) append_velocity_faces(velocity, exact_velocity.set_time(t), faces);
+    check_status(mif_timestep_velocity_tabulated(a.data(), b.data(), c.data(), faces.data(), &forcing, t_n),
+                 velocity.constants.context(), "timestep_velocity");
+  }
   // the library exchanged the storage of velocity and velocity_buffer
   // behind the same field handles; all three now hold device results
   velocity.mark_device_written();
@@ -411,13 +531,18 @@ void solve_pressure_equation_homogeneous_periodic(StaggeredTensor &pressure, Pre
 void solve_pressure_equation_non_homogeneous_neumann(StaggeredTensor &pressure, PressureTensor &,
                                                      const VelocityTensor &velocity,
                                                      const VectorFunction &exact_pressure_gradient, Real dt) {
-  const mif_function gp = require_descriptor(exact_pressure_gradient.has_descriptor(),
-                                             exact_pressure_gradient.descriptor(),
-                                             "solve_pressure_equation_non_homogeneous_neumann");
   auto v = velocity.device();
-  check_status(mif_solve_pressure_non_homogeneous_neumann(pressure.device(), v.data(), &gp,
-                                                         exact_pressure_gradient.time(), dt),
-               pressure.constants.context(), "solve_pressure_equation_non_homogeneous_neumann");
+  if (exact_pressure_gradient.has_descriptor()) {
+    const mif_function gp = exact_pressure_gradient.descriptor();
+    check_status(mif_solve_pressure_non_homogeneous_neumann(pressure.device(), v.data(), &gp,
+                                                           exact_pressure_gradient.time(), dt),
+                 pressure.constants.context(), "solve_pressure_equation_non_homogeneous_neumann");
+  } else {
+    std::vector<double> neumann;
+    append_neumann_faces(pressure, exact_pressure_gradient, neumann);
+    check_status(mif_solve_pressure_non_homogeneous_neumann_tabulated(pressure.device(), v.data(), neumann.data(), dt),
+                 pressure.constants.context(), "solve_pressure_equation_non_homogeneous_neumann");
+  }
   pressure.mark_device_written();
 }
 

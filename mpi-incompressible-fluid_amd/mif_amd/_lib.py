@@ -75,6 +75,7 @@ _V3 = ctypes.c_void_p * 3
 
 SIGNATURES = {
     "mif_ctx_create": [ctypes.POINTER(MifGrid), _I, _I, ctypes.POINTER(_VP)],
+    "mif_ctx_create_cell_centred": [ctypes.POINTER(MifGrid), ctypes.POINTER(_I), _I, _I, ctypes.POINTER(_VP)],
     "mif_ctx_destroy": [_VP],
     "mif_ctx_synchronize": [_VP],
     "mif_last_error": [_VP],
@@ -117,6 +118,12 @@ SIGNATURES = {
     "mif_field_dot": [_VP, _VP, _DP],
     "mif_field_gather": [_VP, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, _DP],
     "mif_field_gather_noalloc": [_VP, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, _DP],
+    "mif_bc_table_size": [_VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
+    "mif_velocity_apply_bc_tabulated": [_V3, _DP],
+    "mif_timestep_tabulated": [_V3, _V3, _V3, _DP, _D, _VP, _VP],
+    "mif_timestep_nhn_tabulated": [_V3, _V3, _V3, _DP, _DP, _D, _VP, _VP],
+    "mif_timestep_velocity_tabulated": [_V3, _V3, _V3, _DP, _FP, _D],
+    "mif_solve_pressure_non_homogeneous_neumann_tabulated": [_VP, _V3, _DP, _D],
     "mif_ctx_set_timing": [_VP, _I],
     "mif_ctx_kernel_time": [_VP, _I, _DP, _U64P],
     "mif_kernel_class_name": [_I],

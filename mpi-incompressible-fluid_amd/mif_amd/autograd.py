@@ -53,13 +53,26 @@ class Rollout:
         # adds a device synchronisation per checkpoint interval)
         self.timed = False
         self.last_backward = {"recompute_s": 0.0, "adjoint_s": 0.0, "recompute_steps": 0, "adjoint_steps": 0}
+        # forwards run so far: a graph node may only replay the latest one,
+        # whose checkpoints and records the Rollout holds
+        self.generation = 0
+        self.force = (0.0, 0.0, 0.0)
 
     # -- device transfers with torch tensors -----------------------------------
     def _load(self, vf: VelocityField, tensors):
-        torch.cuda.synchronize()
+        # contiguous copies first (torch queues them on its current stream for
+        # non-contiguous tensors, e.g. the expanded gradient of a sum), then
+        # wait for torch's work, then the library reads the storage on its own
+        # stream
+        dense = []
         for f, t in zip(vf.components, tensors):
             t = t.detach().contiguous()
-            assert t.dtype == torch.float64 and t.is_cuda and tuple(t.shape) == f.shape
+            if t.dtype != torch.float64 or not t.is_cuda or tuple(t.shape) != f.shape:
+                raise ValueError(f"expected a float64 CUDA tensor of shape {f.shape}, got {t.dtype} "
+                                 f"{'cuda' if t.is_cuda else 'cpu'} {tuple(t.shape)}")
+            dense.append(t)
+        torch.cuda.synchronize()
+        for f, t in zip(vf.components, dense):
             f.upload_device(t.data_ptr())
 
     def _store(self, vf: VelocityField):
@@ -87,7 +100,9 @@ class Rollout:
         timestep(self.vel, y2, y3, self.ev, s * self.ctx.dt, self.p, self.pb)
 
     def run_forward(self, u0, f):
-        self.ctx.set_body_force([float(x) for x in f])
+        self.generation += 1
+        self.force = tuple(float(x) for x in f)
+        self.ctx.set_body_force(self.force)
         self._load(self.vel, u0)
         self._zero(self.p)
         last = (len(self.cp_u) - 1) * self.k
@@ -104,7 +119,14 @@ class Rollout:
         self.ctx.synchronize()
         return self._store(self.vel)
 
-    def run_backward(self, grad_uT):
+    def run_backward(self, grad_uT, generation=None):
+        """Adjoint of the latest forward; `generation` (the forward's number)
+        is checked against it when given."""
+        if generation is not None and generation != self.generation:
+            raise RuntimeError("Rollout: another forward ran on this Rollout after the one being differentiated; "
+                               "its checkpoints are gone (use one Rollout per graph, or call backward first)")
+        # the forward's body force (the context may serve other rollouts)
+        self.ctx.set_body_force(self.force)
         self._load(self.ubar, grad_uT)
         self._zero(self.pbar)
         adjoint_force_take(self.ctx)  # start from a zero accumulator
@@ -150,7 +172,10 @@ class _RolloutFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx_, roll: Rollout, f, u, v, w):
         ctx_.roll = roll
+        f = torch.as_tensor(f)
+        ctx_.f_device, ctx_.f_dtype = f.device, f.dtype
         out = roll.run_forward((u, v, w), f.detach().cpu().numpy())
+        ctx_.generation = roll.generation
         return tuple(out)
 
     @staticmethod
@@ -159,6 +184,6 @@ class _RolloutFn(torch.autograd.Function):
         grads = []
         for g, shp in zip((gu, gv, gw), roll.shapes):
             grads.append(torch.zeros(shp, dtype=torch.float64, device="cuda") if g is None else g)
-        gu0, fbar = roll.run_backward(grads)
-        gf = torch.tensor(fbar, dtype=torch.float64)
+        gu0, fbar = roll.run_backward(grads, generation=ctx_.generation)
+        gf = torch.tensor(fbar, dtype=ctx_.f_dtype, device=ctx_.f_device)
         return (None, gf, *gu0)

@@ -29,7 +29,7 @@ CXX_DIR = os.path.join(PKG_DIR, "cxx")
 CXX_SOURCES = ["shim.cpp", "functions.cpp", "InputParser.cpp", "VTKDatExport.cpp"]
 CXX_LIB = os.path.join(OUT, "libmif_cxx.a")
 DRIVER_DIR = os.path.join(ROOT, "tests", "cpp")
-DRIVERS = ["full_test", "velocity_test", "pressure_test", "io_test"]
+DRIVERS = ["full_test", "velocity_test", "pressure_test", "io_test", "callback_test"]
 # programs of the package itself (the reference's `mif` executable)
 PROGRAMS = {"mif": "mif_main.cpp"}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra",

@@ -19,6 +19,7 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib as L
+from . import boundary as _bd
 from .restart import format_text_dump
 
 
@@ -83,12 +84,21 @@ class GridSpec:
 class Context:
     """One domain on one GPU (Constants + PressureSolverStructures)."""
 
-    def __init__(self, grid: GridSpec, precision: int = L.FP64, device: int = -1):
+    def __init__(self, grid: GridSpec, precision: int = L.FP64, device: int = -1, cell_centred=None):
+        """cell_centred: per-axis flags of cell-centred Neumann axes, for a
+        context of stand-alone Poisson solves (mif_ctx_create_cell_centred):
+        N_g - 1 unknowns at the cell centres, DCT-II/DCT-III transforms."""
         self.lib = L.load()
         self.grid = grid
         self.precision = precision
+        self.cell_centred = tuple(bool(c) for c in cell_centred) if cell_centred is not None else (False,) * 3
         ptr = ctypes.c_void_p()
-        _check(self.lib.mif_ctx_create(ctypes.byref(grid.to_c()), precision, device, ctypes.byref(ptr)), None)
+        if any(self.cell_centred):
+            flags = (ctypes.c_int * 3)(*[1 if c else 0 for c in self.cell_centred])
+            _check(self.lib.mif_ctx_create_cell_centred(ctypes.byref(grid.to_c()), flags, precision, device,
+                                                        ctypes.byref(ptr)), None)
+        else:
+            _check(self.lib.mif_ctx_create(ctypes.byref(grid.to_c()), precision, device, ctypes.byref(ptr)), None)
         self.ptr = ptr
         dt = ctypes.c_double()
         ext = (ctypes.c_uint64 * 3)()
@@ -157,7 +167,7 @@ class Context:
     def npts(self):
         g = self.grid
         Ng = (g.Nx_global, g.Ny_global, g.Nz_global)
-        return tuple(Ng[a] - (1 if g.periodic[a] else 0) for a in range(3))
+        return tuple(Ng[a] - (1 if g.periodic[a] or self.cell_centred[a] else 0) for a in range(3))
 
     def poisson_set_rhs(self, b: np.ndarray):
         n = self.npts
@@ -349,7 +359,13 @@ class VelocityField:
     def download(self):
         return tuple(f.download() for f in self.components)
 
-    def apply_bc(self, exact_velocity: L.MifFunction, t: float):
+    def apply_bc(self, exact_velocity, t: float):
+        """VelocityTensor::apply_bc; exact_velocity is a descriptor or three
+        callables f(t, x, y, z) evaluated on the host into a face table."""
+        if _bd.is_callables(exact_velocity):
+            tab = _bd.velocity_face_table(self.ctx.grid, exact_velocity, t)
+            self.ctx.check(self.ctx.lib.mif_velocity_apply_bc_tabulated(self.c_array(), _dptr(tab)))
+            return
         self.ctx.check(self.ctx.lib.mif_velocity_apply_bc(self.c_array(), ctypes.byref(exact_velocity), t))
 
     def swap_data(self, other: "VelocityField"):
@@ -366,18 +382,47 @@ class VelocityField:
 
 # ---- reference-named free functions -------------------------------------------
 
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
 def timestep(velocity: VelocityField, velocity_buffer: VelocityField, velocity_buffer_2: VelocityField,
-             exact_velocity: L.MifFunction, t_n: float, pressure: Field, pressure_buffer: Field):
-    """timestep (include/Timestep.h:16-19)."""
+             exact_velocity, t_n: float, pressure: Field, pressure_buffer: Field):
+    """timestep (include/Timestep.h:16-19).  exact_velocity is a descriptor
+    (function()) or three callables f(t, x, y, z), evaluated on the host at
+    the stage times into face tables."""
     ctx = velocity.ctx
+    if _bd.is_callables(exact_velocity):
+        times = _bd.full_step_times(ctx.dt, t_n)
+        faces = np.concatenate([_bd.velocity_face_table(ctx.grid, exact_velocity, t) for t in times[1:]])
+        ctx.check(ctx.lib.mif_timestep_tabulated(velocity.c_array(), velocity_buffer.c_array(),
+                                                 velocity_buffer_2.c_array(), _dptr(faces), t_n, pressure.ptr,
+                                                 pressure_buffer.ptr))
+        return
     ctx.check(ctx.lib.mif_timestep(velocity.c_array(), velocity_buffer.c_array(), velocity_buffer_2.c_array(),
                                    ctypes.byref(exact_velocity), t_n, pressure.ptr, pressure_buffer.ptr))
 
 
 def timestep_nhn(velocity, velocity_buffer, velocity_buffer_2, exact_velocity, exact_pressure_gradient, t_n,
                  pressure, pressure_buffer):
-    """timestep_nhn (include/Timestep.h:23-27)."""
+    """timestep_nhn (include/Timestep.h:23-27).  Descriptors, or (both) three
+    callables each: the Neumann data of stage s are then
+    get_difference_over_time(new_time, prev_time) of the gradient
+    (src/Timestep.cpp:89-95), tabulated on the host."""
     ctx = velocity.ctx
+    if _bd.is_callables(exact_velocity) or _bd.is_callables(exact_pressure_gradient):
+        if not (_bd.is_callables(exact_velocity) and _bd.is_callables(exact_pressure_gradient)):
+            raise TypeError("timestep_nhn: give both functions as descriptors or both as callables")
+        times = _bd.full_step_times(ctx.dt, t_n)
+        faces = np.concatenate([_bd.velocity_face_table(ctx.grid, exact_velocity, t) for t in times[1:]])
+        neumann = np.concatenate([
+            _bd.neumann_face_table(ctx.grid, _bd.difference_over_time(exact_pressure_gradient, times[s],
+                                                                      times[s - 1]), 0.0)
+            for s in (1, 2, 3)])
+        ctx.check(ctx.lib.mif_timestep_nhn_tabulated(velocity.c_array(), velocity_buffer.c_array(),
+                                                     velocity_buffer_2.c_array(), _dptr(faces), _dptr(neumann), t_n,
+                                                     pressure.ptr, pressure_buffer.ptr))
+        return
     ctx.check(ctx.lib.mif_timestep_nhn(velocity.c_array(), velocity_buffer.c_array(), velocity_buffer_2.c_array(),
                                        ctypes.byref(exact_velocity), ctypes.byref(exact_pressure_gradient), t_n,
                                        pressure.ptr, pressure_buffer.ptr))
@@ -385,8 +430,17 @@ def timestep_nhn(velocity, velocity_buffer, velocity_buffer_2, exact_velocity, e
 
 def timestep_velocity(velocity, velocity_buffer, rhs_buffer, exact_velocity, forcing, t_n):
     """timestep_velocity (include/TimestepVelocity.h:15-16); swaps velocity and
-    velocity_buffer storage like the reference."""
+    velocity_buffer storage like the reference.  exact_velocity is a
+    descriptor or three callables (host face tables); the forcing is the
+    manufactured forcing descriptor, as the reference hard-wires it."""
     ctx = velocity.ctx
+    if _bd.is_callables(exact_velocity):
+        faces = np.concatenate([_bd.velocity_face_table(ctx.grid, exact_velocity, t)
+                                for t in _bd.velocity_step_times(ctx.dt, t_n)])
+        ctx.check(ctx.lib.mif_timestep_velocity_tabulated(velocity.c_array(), velocity_buffer.c_array(),
+                                                          rhs_buffer.c_array(), _dptr(faces), ctypes.byref(forcing),
+                                                          t_n))
+        return
     ctx.check(ctx.lib.mif_timestep_velocity(velocity.c_array(), velocity_buffer.c_array(), rhs_buffer.c_array(),
                                             ctypes.byref(exact_velocity), ctypes.byref(forcing), t_n))
 
@@ -398,8 +452,14 @@ def solve_pressure_equation_homogeneous_periodic(pressure: Field, velocity: Velo
 
 
 def solve_pressure_equation_non_homogeneous_neumann(pressure, velocity, exact_pressure_gradient, t, dt):
-    """include/PressureEquation.h:16-21; the gradient is evaluated at time t."""
+    """include/PressureEquation.h:16-21; the gradient (a descriptor, or three
+    callables tabulated on the host) is evaluated at time t."""
     ctx = pressure.ctx
+    if _bd.is_callables(exact_pressure_gradient):
+        tab = _bd.neumann_face_table(ctx.grid, exact_pressure_gradient, t)
+        ctx.check(ctx.lib.mif_solve_pressure_non_homogeneous_neumann_tabulated(pressure.ptr, velocity.c_array(),
+                                                                              _dptr(tab), dt))
+        return
     ctx.check(ctx.lib.mif_solve_pressure_non_homogeneous_neumann(pressure.ptr, velocity.c_array(),
                                                                 ctypes.byref(exact_pressure_gradient), t, dt))
 

@@ -71,3 +71,13 @@ def test_invalid_grid_rejected_before_device_use():
     with pytest.raises(mif_amd.MifError) as ei:
         mif_amd.Context(mif_amd.GridSpec(8, 8, 8, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 1))
     assert ei.value.status == 1
+    # a wall axis of 2050 nodes: complex length 4098 has no specialised
+    # transform, and the generic kernel would need 128 KB of LDS per workgroup
+    with pytest.raises(mif_amd.MifError) as ei:
+        mif_amd.Context(mif_amd.GridSpec(2050, 8, 8, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1))
+    assert ei.value.status == 5  # MIF_ERR_UNSUPPORTED
+    assert b"LDS" in mif_amd.load().mif_last_error(None)
+    # 2049 periodic nodes (length 2048, 64 KB) pass the check and reach the device query
+    with pytest.raises(mif_amd.MifError) as ei:
+        mif_amd.Context(mif_amd.GridSpec(2049, 8, 8, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 1.0, 1.0, 1, (True, False, False)))
+    assert ei.value.status == 6  # MIF_ERR_NO_DEVICE on a host without a GPU
