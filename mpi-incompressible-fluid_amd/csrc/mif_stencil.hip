@@ -3,6 +3,8 @@
 // Each kernel cites the reference loop it replaces.  All kernels take the
 // padded common layout of mif_device.h; T is the storage/arithmetic type of
 // the fields (double for the plain form, float for the mixed form).
+#include <type_traits>
+
 #include "mif_kernels.h"
 
 namespace mif {
@@ -136,8 +138,9 @@ constexpr bool kStageVec = true;
 // VEC: halo planes fetched with 16-byte loads (element 1 of every row
 // 16-byte aligned: the layout's storage shift).
 template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = false>
-__global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax,
-                                               int kmax, int ntx, int nty, int ntiles) {
+// (FP64 full step: at most 80 VGPRs, i.e. three 512-thread workgroups per CU)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && KIND <= 3 ? 6 : 1)))
+k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int kmax, int ntx, int nty, int ntiles) {
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 and
   // more (plane k+2 is committed into a free slot while plane k is
   // computed; one barrier per plane)
@@ -147,6 +150,8 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
   // FP32: 181 -> 183 us on 64 x 8 tiles, 176 -> 179 us on 128 x 4, so FP32
   // keeps the load)
   constexpr bool PCARRY = sizeof(T) == 8;
+  // order of the memory requests per plane (see the plane loop)
+  constexpr bool AHEAD = sizeof(T) == 8;
   using TT = StageTile<T>;
   constexpr int STX = TT::STX, STY = TT::STY, SHX = TT::SHX, SHY = TT::SHY, STN = TT::STN;
   constexpr int PLANE_ELEMS = TT::PLANE_ELEMS, PQ = TT::PQ;
@@ -171,10 +176,17 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
   constexpr int NHV = 3 * SHY * 2;                // halo-column elements per plane
   static_assert(NHV <= STN, "one halo-column element per thread");
   constexpr int PR = VEC ? QV * VW + 1 : PQ;      // plane values a thread holds
-  struct alignas(16) Vec {
-    T e[VW];
-  };
-  const T* src[VEC ? QV + 1 : PQ];
+  typedef T Vec __attribute__((ext_vector_type(VW)));
+  // AHEAD, halo loads go through global-address-space pointers: generic
+  // (flat) loads also count as LDS operations, so every wait for the LDS
+  // ring would wait for the plane in flight as well.  (The FP32 kernel, which
+  // waits for its halo plane before the stencil work anyway, measured 1 %
+  // faster with the generic loads.)
+  typedef const __attribute__((address_space(1))) T GT1;
+  typedef const __attribute__((address_space(1))) Vec GV1;
+  using GT = std::conditional_t<AHEAD, GT1, const T>;
+  using GV = std::conditional_t<AHEAD, GV1, const Vec>;
+  GT* src[VEC ? QV + 1 : PQ];
   int dst[VEC ? QV + 1 : PQ];
   bool ok[VEC ? QV + 1 : PQ];
   if constexpr (VEC) {
@@ -188,14 +200,14 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
       // its end (into the next row, or the allocation's tail); those lanes
       // feed no output point
       ok[r] = q < NIV && gi < L.PX && gj < L.SY;
-      src[r] = ok[r] ? X[f] + L.idx(gi, gj, 0) : X[0] + 1;
+      src[r] = (GT*)(ok[r] ? X[f] + L.idx(gi, gj, 0) : X[0] + 1);
       dst[r] = q < NIV ? (f * SHY + jj) * SHX + 1 + VW * iq : -1;
     }
     const int f = tid / (SHY * 2), rr = tid - f * (SHY * 2);
     const int jj = rr >> 1, side = rr & 1;
     const int gi = side ? i0 + STX : i0 - 1, gj = j0 - 1 + jj;
     ok[QV] = tid < NHV && gi < L.PX && gj < L.SY;
-    src[QV] = ok[QV] ? X[f] + L.idx(gi, gj, 0) : X[0];
+    src[QV] = (GT*)(ok[QV] ? X[f] + L.idx(gi, gj, 0) : X[0]);
     dst[QV] = tid < NHV ? (f * SHY + jj) * SHX + (side ? SHX - 1 : 0) : -1;
   } else {
 #pragma unroll
@@ -205,27 +217,30 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
       const int jj = rr / SHX, ii = rr - jj * SHX;
       const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
       ok[r] = q < PLANE_ELEMS && gi < L.PX && gj < L.SY;
-      src[r] = ok[r] ? X[f] + L.idx(gi, gj, 0) : X[0];
+      src[r] = (GT*)(ok[r] ? X[f] + L.idx(gi, gj, 0) : X[0]);
       dst[r] = q < PLANE_ELEMS ? q : -1;
     }
   }
   // every plane read lies in 0 .. kmax + 1 <= SZ - 1, so the load address is
-  // valid for every lane and the loads need no branch
+  // valid for every lane and the loads need no branch.  Lanes outside the
+  // field are zeroed when the values are committed to LDS (AHEAD: so that no
+  // instruction waits for the loads before the commit) or as they arrive.
+  auto mask = [&](bool keep, T x) { return keep ? x : (T)0; };
   auto fetch = [&](int k, T (&v)[PR]) {
     if constexpr (VEC) {
 #pragma unroll
       for (int r = 0; r < QV; r++) {
-        const Vec x = *(const Vec*)(src[r] + (long)k * L.plane);
+        const Vec x = *(GV*)(src[r] + (long)k * L.plane);
 #pragma unroll
-        for (int e = 0; e < VW; e++) v[r * VW + e] = ok[r] ? x.e[e] : (T)0;
+        for (int e = 0; e < VW; e++) v[r * VW + e] = AHEAD ? x[e] : mask(ok[r], x[e]);
       }
       const T h = src[QV][(long)k * L.plane];
-      v[QV * VW] = ok[QV] ? h : (T)0;
+      v[QV * VW] = AHEAD ? h : mask(ok[QV], h);
     } else {
 #pragma unroll
       for (int r = 0; r < PQ; r++) {
         const T x = src[r][(long)k * L.plane];
-        v[r] = ok[r] ? x : (T)0;
+        v[r] = AHEAD ? x : mask(ok[r], x);
       }
     }
   };
@@ -236,12 +251,12 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
       for (int r = 0; r < QV; r++)
         if (dst[r] >= 0)
 #pragma unroll
-          for (int e = 0; e < VW; e++) base[dst[r] + e] = v[r * VW + e];
-      if (dst[QV] >= 0) base[dst[QV]] = v[QV * VW];
+          for (int e = 0; e < VW; e++) base[dst[r] + e] = AHEAD ? mask(ok[r], v[r * VW + e]) : v[r * VW + e];
+      if (dst[QV] >= 0) base[dst[QV]] = AHEAD ? mask(ok[QV], v[QV * VW]) : v[QV * VW];
     } else {
 #pragma unroll
       for (int r = 0; r < PQ; r++)
-        if (dst[r] >= 0) base[dst[r]] = v[r];
+        if (dst[r] >= 0) base[dst[r]] = AHEAD ? mask(ok[r], v[r]) : v[r];
     }
   };
   {
@@ -254,7 +269,7 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
     commit((k0 + 1) % NS, c);
   }
   T pre[PR];
-  if (k0 + 2 <= k1) fetch(k0 + 2, pre);
+  if (!AHEAD && k0 + 2 <= k1) fetch(k0 + 2, pre);
   __syncthreads();
 
   const bool in_tile = i <= imax && j <= jmax;
@@ -262,65 +277,99 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
   const long doff[3] = {1, L.PX, L.plane};
   T p_prev = (T)0;
   if (KIND <= 3 && PCARRY && in_tile) p_prev = ((const T*)s.p)[L.idx(i, j, k0 - 1)];
+  // Order of the memory requests per plane k.
+  // AHEAD: the pointwise operands of plane k first, then the halo plane k+2
+  // into registers; the stencil work of plane k waits for the pointwise
+  // operands only (loads complete in issue order), so plane k+2 stays in
+  // flight during the work and is committed to LDS after it (256^3 FP64
+  // stage kernel: 356 -> 343 us).
+  // Otherwise: plane k+2 is committed and plane k+3 requested before the
+  // pointwise operands of plane k (FP32 on four slots: 175 us, against 181
+  // with the order above).
+  // The pointwise loads are unconditional: threads outside the tile load at
+  // a clamped column (every address lies inside the allocation: i, j, k >= 1
+  // and i <= imax, j <= jmax, k <= kmax) and their values are not used, so
+  // the number of loads in flight is fixed for the compiler's waits.
+  const long oc = L.idx(min(i, imax), min(j, jmax), 0);
   for (int k = k0; k < k1; k++) {
-    if constexpr (NS >= 4) {
+    if constexpr (!AHEAD && NS >= 4) {
       // the slot of plane k-2, which no thread reads any more
       if (k + 2 <= k1) commit((k + 2) % NS, pre);
       if (k + 3 <= k1) fetch(k + 3, pre);
     }
-    if (in_tile) {
-      const long o = L.idx(i, j, k);
-      bool inside[3];
-      for (int c = 0; c < 3; c++) inside[c] = interior(g, c, i, j, k);
-      // pointwise operands of all three components, loaded together
-      T p0 = (T)0, pm[3] = {(T)0, (T)0, (T)0}, av[3] = {(T)0, (T)0, (T)0}, bv[3] = {(T)0, (T)0, (T)0};
+    const long o = L.idx(i, j, k);
+    const long ol = AHEAD ? oc + (long)k * L.plane : o;
+    T p0 = (T)0, pm[3] = {(T)0, (T)0, (T)0}, av[3] = {(T)0, (T)0, (T)0}, bv[3] = {(T)0, (T)0, (T)0};
+    // pointwise operands of all three components, loaded together
+    auto load_pointwise = [&](const bool (&inside)[3]) {
       if constexpr (KIND <= 3) {
-        const T* p = (const T*)s.p + o;
+        const T* p = (const T*)s.p + ol;
         p0 = p[0];
 #pragma unroll
         for (int c = 0; c < 2; c++)
           if (inside[c]) pm[c] = p[-doff[c]];
-        // p at plane k-1 of this column: the previous plane's p0 (loaded
-        // before the loop for the first plane)
-        if constexpr (PCARRY) {
-          pm[2] = p_prev;
-          p_prev = p0;
-        } else if (inside[2]) {
-          pm[2] = p[-doff[2]];
-        }
+        if constexpr (!PCARRY)
+          if (inside[2]) pm[2] = p[-doff[2]];
         if constexpr (KIND >= 2) {
 #pragma unroll
           for (int c = 0; c < 3; c++)
-            if (inside[c]) av[c] = load_in<NT && kNtLoads>(&((const T*)s.A[c])[o]);
+            if (inside[c]) av[c] = load_in<NT && kNtLoads>(&((const T*)s.A[c])[ol]);
         }
       } else if constexpr (KIND >= 5) {
 #pragma unroll
         for (int c = 0; c < 3; c++)
           if (inside[c]) {
-            av[c] = ((const T*)vs.R[c])[o];
-            if constexpr (KIND == 5) bv[c] = ((const T*)vs.V[c])[o];
+            av[c] = ((const T*)vs.R[c])[ol];
+            if constexpr (KIND == 5) bv[c] = ((const T*)vs.V[c])[ol];
           }
+      }
+    };
+    if constexpr (AHEAD) {
+      const bool every[3] = {true, true, true};
+      load_pointwise(every);
+    }
+    // plane k+2 is read by plane k+1's stencil; past the chunk's last plane
+    // the fetch repeats plane k1 (a cache hit) and is not committed
+    const bool more = k + 2 <= k1;
+    if constexpr (AHEAD) fetch(min(k + 2, k1), pre);
+    // AHEAD, the full step computes every component on every thread (the
+    // taps of a thread outside the field still lie in the LDS tile; only the
+    // stores are masked), so that the compiler keeps the loads above
+    // unconditional instead of sinking them into the branches
+    constexpr bool EVERY = AHEAD && KIND <= 3;
+    if (EVERY || in_tile) {
+      bool inside[3];
+      for (int c = 0; c < 3; c++) inside[c] = interior(g, c, i, j, k);
+      if constexpr (!AHEAD) load_pointwise(inside);
+      // p at plane k-1 of this column: the previous plane's p0 (loaded
+      // before the loop for the first plane)
+      if constexpr (KIND <= 3 && PCARRY) {
+        pm[2] = p_prev;
+        p_prev = p0;
       }
       const int slot[3] = {(k + NS - 1) % NS, k % NS, (k + 1) % NS};
       auto tap = [&](int f, int di, int dj, int dk) -> T { return pl[slot[dk + 1]][f][ty + 1 + dj][tx + 1 + di]; };
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        if (!inside[c]) continue;
+        if (!EVERY && !inside[c]) continue;
         T rhs = c == 0 ? rhs_u<T>(g, tap) : (c == 1 ? rhs_v<T>(g, tap) : rhs_w<T>(g, tap));
         const T x = tap(c, 0, 0, 0);
         if constexpr (KIND <= 3) {
           rhs += (T)s.force[c];
           // pressure_gradient_{u,v,w} (include/PressureGradient.h:9-25)
           const T pg = (p0 - pm[c]) * (T)g.one_over_h[c];
+          T o1, o2 = rhs;
           if constexpr (KIND == 1) {
-            store_out<NT>(&((T*)s.O1[c])[o], x + (T)s.a1 * rhs - (T)s.b * pg);
-            store_out<NT>(&((T*)s.O2[c])[o], rhs);
+            o1 = x + (T)s.a1 * rhs - (T)s.b * pg;
           } else if constexpr (KIND == 2) {
-            const T rhs_2_scaled = (T)s.a2 * rhs;
-            store_out<NT>(&((T*)s.O1[c])[o], x + (T)s.a1 * av[c] + rhs_2_scaled - (T)s.b * pg);
-            store_out<NT>(&((T*)s.O2[c])[o], rhs_2_scaled);
+            o2 = (T)s.a2 * rhs;
+            o1 = x + (T)s.a1 * av[c] + o2 - (T)s.b * pg;
           } else {
-            store_out<NT>(&((T*)s.O1[c])[o], x + (-av[c]) + (T)s.a2 * rhs - (T)s.b * pg);
+            o1 = x + (-av[c]) + (T)s.a2 * rhs - (T)s.b * pg;
+          }
+          if (!EVERY || inside[c]) {
+            store_out<NT>(&((T*)s.O1[c])[o], o1);
+            if constexpr (KIND <= 2) store_out<NT>(&((T*)s.O2[c])[o], o2);
           }
         } else {
           // calculate_momentum_rhs_with_forcing_* (include/MomentumEquationForcing.h:10-31)
@@ -349,12 +398,21 @@ __global__ void __launch_bounds__(512) k_stage(DGrid g, Layout L, StageArgs s, V
         }
       }
     }
-    __syncthreads();
-    if constexpr (NS == 3) {
-      // plane k+2 replaces plane k-1; plane k+3 starts loading
-      if (k + 2 <= k1) commit((k + 2) % NS, pre);
-      if (k + 3 <= k1) fetch(k + 3, pre);
+    // plane k+2 goes to the slot of plane k-1 (three slots: once every
+    // thread is done with plane k) or of plane k-2 (four and more slots: no
+    // thread has read it since the previous barrier)
+    if constexpr (AHEAD) {
+      if constexpr (NS == 3) __syncthreads();
+      if (more) commit((k + 2) % NS, pre);
       __syncthreads();
+    } else {
+      __syncthreads();
+      if constexpr (NS == 3) {
+        // plane k+2 replaces plane k-1; plane k+3 starts loading
+        if (more) commit((k + 2) % NS, pre);
+        if (k + 3 <= k1) fetch(k + 3, pre);
+        __syncthreads();
+      }
     }
   }
 }
