@@ -367,7 +367,16 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
           } else {
             o1 = x + (-av[c]) + (T)s.a2 * rhs - (T)s.b * pg;
           }
-          if (!EVERY || inside[c]) {
+          if constexpr (EVERY) {
+            // unconditional stores, so that the loop has no branches: a
+            // point outside component c's interior is a boundary-shell
+            // point, which the boundary pass rewrites after the stage (O1)
+            // or nothing reads (O2); threads outside the tile store at the
+            // shell point (0, 0, k)
+            const long os = in_tile ? o : L.idx(0, 0, k);
+            store_out<NT>(&((T*)s.O1[c])[os], o1);
+            if constexpr (KIND <= 2) store_out<NT>(&((T*)s.O2[c])[os], o2);
+          } else {
             store_out<NT>(&((T*)s.O1[c])[o], o1);
             if constexpr (KIND <= 2) store_out<NT>(&((T*)s.O2[c])[o], o2);
           }
