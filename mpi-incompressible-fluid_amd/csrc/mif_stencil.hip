@@ -124,9 +124,13 @@ struct StageTile {
   static constexpr int PQ = (PLANE_ELEMS + STN - 1) / STN;   // plane elements per thread
   static_assert(STN == 512, "the stage kernel's launch bounds");
 };
-// planes per workgroup (FP32 stage kernel on 128 x 4 tiles at 256^3: 8
-// planes 180.4 us, 16 176.5 us, 24 178.0 us, 32 192.5 us)
-constexpr int SKB = 16;
+// planes per workgroup.  FP32 on 128 x 4 tiles at 256^3: 8 planes 180.4 us,
+// 16 176.5 us, 24 178.0 us, 32 192.5 us.  FP64 on 64 x 8 tiles, with the
+// plane order of k_stage's AHEAD form: 4 planes 329 us, 6 328, 8 320, 10 320,
+// 12 323, 16 336, 24 357, 32 342 (four ring slots instead of three: 8 planes
+// 318 us, 16 330 us, so three slots stay).
+template <typename T>
+constexpr int stage_planes() { return sizeof(T) == 8 ? 8 : 16; }
 
 // ring slots of the FP32 and FP64 stage kernels (see k_stage)
 constexpr int kStageSlots32 = 4, kStageSlots64 = 3;
@@ -159,6 +163,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * STX + tx;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
+  constexpr int SKB = stage_planes<T>();
   const int i0 = bx * STX + 1, j0 = by * STY + 1, k0 = bz * SKB + 1;
   const int k1 = min(k0 + SKB, kmax + 1);
   const int i = i0 + tx, j = j0 + ty;
@@ -938,6 +943,7 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
   const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
   constexpr int STX = StageTile<T>::STX, STY = StageTile<T>::STY;
+  constexpr int SKB = stage_planes<T>();
   const int ntx = (imax + STX - 1) / STX, nty = (jmax + STY - 1) / STY, ntz = (kmax + SKB - 1) / SKB;
   const int ntiles = ntx * nty * ntz;
   // the 16-byte loads need element 1 of every row on a 16-byte boundary:
