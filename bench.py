@@ -433,11 +433,11 @@ def stream_ceiling(cls: str, res, roof):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_channel_256cubed.md")
 CLASS_KERNELS = {
     "stage": r"k_stage<{T}, [123](, \w+)*>",
-    "fft_x_div": r"k_fft_p2<{T}, \d+, \d+, 0, 0, true(, \w+)?>",
-    "fft_x_unpack": r"k_fft_p2<{T}, \d+, \d+, 0, 1, true(, \w+)?>",
-    "fft_y": r"k_fft_p2<{T}, \d+, \d+, 1, [01], false(, \w+)?>",
-    "fft_z_fused": r"k_fft_p2<{T}, \d+, \d+, 2, 2, false(, \w+)?>",
-    "project": r"k_project<{T}(, \w+)?>",
+    "fft_x_div": r"k_fft_p2<{T}, \d+, \d+, 0, 0, true, \w+, false>",
+    "fft_x_unpack": r"k_fft_p2<{T}, \d+, \d+, 0, 1, true, \w+, false>",
+    "fft_y": r"k_fft_p2<{T}, \d+, \d+, 1, [01], false(, \w+)*>",
+    "fft_z_fused": r"k_fft_p2<{T}, \d+, \d+, 2, 2, false(, \w+)*>",
+    "project": r"k_project<{T}(, \w+)*>",
 }
 
 

@@ -350,19 +350,17 @@ __device__ inline void adj_xb_point(const DGrid& g, long o, int i, int j, int k,
 // the z_bar, p_bar accesses go to global memory.  Ring cells outside the
 // array hold zero: wherever N_bar is non-zero (component interiors) every X
 // tap of its term is an array point, so those zeros only meet N_bar = 0.
-#ifndef MIF_ADJ_KB
-#define MIF_ADJ_KB 4
-#endif
-#ifndef MIF_ADJ_TILED
-#define MIF_ADJ_TILED 1
-#endif
-constexpr int AXT = 64, AYT = 4, AKB = MIF_ADJ_KB;
+// (256^3 FP64, 8 adjoint steps: 64 x 4 tiles of 4 planes 54.9 ms, 8 planes
+// 53.9, 16 planes 53.7; 64 x 8 tiles 55.8 (4 planes) and 55.1 (8 planes):
+// the kernel is not bound by its halo loads.  Every tap read from global
+// memory instead of the ring measured slower.)
+constexpr int AXT = 64, AYT = 4, AKB = 4;
 constexpr int AHX = AXT + 2, AHY = AYT + 2;
 constexpr int APL = AHY * AHX;                         // one field of a ring plane
 constexpr int ARH = (APL + AXT * AYT - 1) / (AXT * AYT);  // elements per thread and field
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_xb(DGrid g, Layout L, AdjVec Xv, AdjVec Nbv, AdjVec Zv, AdjVec Xbv,
+__global__ void __launch_bounds__(AXT * AYT) k_adj_xb(DGrid g, Layout L, AdjVec Xv, AdjVec Nbv, AdjVec Zv, AdjVec Xbv,
                                                 T* __restrict__ P, double bcoef, int ntx, int nty, int ntiles) {
   __shared__ T ring[3][6][AHY][AHX];
   const int tx = threadIdx.x % AXT, ty = threadIdx.x / AXT, tid = threadIdx.x;
@@ -386,25 +384,25 @@ __global__ void __launch_bounds__(256) k_adj_xb(DGrid g, Layout L, AdjVec Xv, Ad
     off[h] = ok[h] ? (int)L.idx(gi, gj, 0) : 0;
     lds[h] = q < APL ? q : -1;
   }
+  // planes outside the array are fetched at plane 0 and zeroed, like the
+  // cells outside it, when committed (so no instruction waits for the loads
+  // before the commit)
   auto fetch = [&](int k, T (&v)[6][ARH]) {
-    const bool kin = k >= 0 && k < L.SZ;
-    const long kk = kin ? (long)k * L.plane : 0;
+    const long kk = k >= 0 && k < L.SZ ? (long)k * L.plane : 0;
 #pragma unroll
     for (int f = 0; f < 6; f++)
 #pragma unroll
-      for (int h = 0; h < ARH; h++) {
-        const T x = F[f][kk + off[h]];
-        v[f][h] = (ok[h] && kin) ? x : (T)0;
-      }
+      for (int h = 0; h < ARH; h++) v[f][h] = F[f][kk + off[h]];
   };
   auto slot_of = [](int k) { return (k + 3) % 3; };
   auto commit = [&](int k, const T (&v)[6][ARH]) {
     const int sl = slot_of(k);
+    const bool kin = k >= 0 && k < L.SZ;
 #pragma unroll
     for (int f = 0; f < 6; f++)
 #pragma unroll
       for (int h = 0; h < ARH; h++)
-        if (lds[h] >= 0) (&ring[sl][f][0][0])[lds[h]] = v[f][h];
+        if (lds[h] >= 0) (&ring[sl][f][0][0])[lds[h]] = ok[h] && kin ? v[f][h] : (T)0;
   };
   {
     T a[6][ARH];
@@ -448,34 +446,6 @@ __global__ void __launch_bounds__(256) k_adj_xb(DGrid g, Layout L, AdjVec Xv, Ad
   }
 }
 
-// The same point update with every tap read from global memory (clamped
-// indices), one thread per point over 256-position slices of the padded
-// plane.
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_xb_direct(DGrid g, Layout L, AdjVec Xv, AdjVec Nbv, AdjVec Zv,
-                                                       AdjVec Xbv, T* __restrict__ P, double bcoef, int nqt) {
-  const int qt = blockIdx.x % nqt, k = blockIdx.x / nqt;
-  const int q = qt * 256 + threadIdx.x;
-  const int i = q % L.PX, j = q / L.PX;
-  if (j > g.N[1] || i > g.N[0] || k > g.N[2]) return;
-  const T* F[6] = {(const T*)Xv.f[0], (const T*)Xv.f[1], (const T*)Xv.f[2],
-                   (const T*)Nbv.f[0], (const T*)Nbv.f[1], (const T*)Nbv.f[2]};
-  const long o = L.idx(i, j, k);
-  const long last = L.elems - 1;
-  const long PX = L.PX, PL = L.plane;
-  auto tp = [&](int f, int di, int dj, int dk) -> double {
-    long idx = o + di + PX * dj + PL * dk;
-    idx = idx < 0 ? 0 : (idx > last ? last : idx);
-    return (double)F[f][idx];
-  };
-  auto zt = [&](int c, int di, int dj, int dk) -> double {
-    long idx = o + di + PX * dj + PL * dk;
-    idx = idx < 0 ? 0 : (idx > last ? last : idx);
-    return (double)((const T*)Zv.f[c])[idx];
-  };
-  adj_xb_point<T>(g, o, i, j, k, tp, zt, (T*)Xbv.f[0], (T*)Xbv.f[1], (T*)Xbv.f[2], P, bcoef);
-}
-
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -502,16 +472,11 @@ void launch_adj_sum(const double* partial, int nblk, int accumulate, double* out
 template <typename T>
 void launch_adj_xb(const DGrid& g, const Layout& L, const AdjVec& X, const AdjVec& Nb, const AdjVec& Zb,
                    const AdjVec& Xb, T* P, double bcoef, hipStream_t st) {
-#if MIF_ADJ_TILED
   // tiles over points 0..N per axis
   const int ntx = (g.N[0] + 1 + AXT - 1) / AXT, nty = (g.N[1] + 1 + AYT - 1) / AYT;
   const int ntz = (g.N[2] + 1 + AKB - 1) / AKB;
   const int ntiles = ntx * nty * ntz;
   k_adj_xb<T><<<ntiles, AXT * AYT, 0, st>>>(g, L, X, Nb, Zb, Xb, P, bcoef, ntx, nty, ntiles);
-#else
-  const int nqt = (int)((L.plane + 255) / 256);
-  k_adj_xb_direct<T><<<(unsigned)(nqt * (g.N[2] + 1)), 256, 0, st>>>(g, L, X, Nb, Zb, Xb, P, bcoef, nqt);
-#endif
 }
 
 #define MIF_ADJ_INST(T)                                                                                              \

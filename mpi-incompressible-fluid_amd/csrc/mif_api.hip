@@ -964,17 +964,15 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
   c->L.SZ = maxz;
   c->L.plane = (long)c->L.PX * c->L.SY;
   c->L.elems = c->L.plane * c->L.SZ;
-#ifndef MIF_FIELD_SHIFT
-#define MIF_FIELD_SHIFT 1
-#endif
-  // Measured step times with and without the shift (FP32/FP64 fields of
+  // Storage shift: element 1 of every row on a 128-byte line, so that the
+  // interior of a row starts a cache line.  Measured step times with and without the shift (FP32/FP64 fields of
   // 67/135 MB at 256^3: 1.728 -> 1.701 ms mixed, 3.303 -> 3.286 ms FP64;
   // 1.09 GB FP64 fields at 512^3: 24.80 -> 25.06 ms; 1.8 GB FP32 fields at
   // 768^3: 57.98 -> 58.85 ms): the shift pays while a field fits the 256 MB
   // Infinity Cache, so it is applied up to that size
   constexpr long kShiftMaxFieldBytes = 256L << 20;
   const bool cached = c->L.elems * (long)c->esize <= kShiftMaxFieldBytes;
-  const bool shift = MIF_FIELD_SHIFT && cached;
+  const bool shift = cached;
   c->soff = shift ? 128 / (long)c->esize - 1 : 0;
   c->L.lead = (int)c->soff;
   // Non-temporal stores (and loads of single-use operands) in the stage and
@@ -989,15 +987,9 @@ int ctx_init(mif_ctx* c, const mif_grid* grid, const uint64_t Ng[3], int precisi
   // non-temporal access took the 128^3 FP64 step (18 MB fields) from 0.52 to
   // 0.62 ms and the mixed one (9 MB) from 0.38 to 0.44 ms.  A 192^3 FP32
   // field (31 MB) measured the same either way.
-#ifndef MIF_NT_STORES
-#define MIF_NT_STORES 1
-#endif
-#ifndef MIF_NT_MIN_FIELD_MB
-#define MIF_NT_MIN_FIELD_MB 32
-#endif
-  constexpr long kNtMinFieldBytes = (long)MIF_NT_MIN_FIELD_MB << 20;
+  constexpr long kNtMinFieldBytes = 32L << 20;
   const bool streamed = c->L.elems * (long)c->esize > kNtMinFieldBytes;
-  c->L.nt = MIF_NT_STORES && cached && streamed ? 1 : 0;
+  c->L.nt = cached && streamed ? 1 : 0;
   // kernels decompose flat indices in 32 bits
   if (c->L.elems >= (1L << 31))
     return fail(nullptr, MIF_ERR_UNSUPPORTED, "grid too large: a field would hold %ld padded elements (limit 2^31 - 1)",

@@ -88,10 +88,6 @@ __device__ inline void dft(Cx<T>* v) {
   }
 }
 
-#ifndef MIF_TW_POW
-#define MIF_TW_POW 1
-#endif
-
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 // L = 2^m or 3 * 2^m: stages of radix 8, 8, ..., then 4 or 2, then 3
 constexpr bool has3(int L) { return L % 3 == 0; }
@@ -121,15 +117,14 @@ __device__ constexpr inline int pd(int e) { return e + (e >> 3); }
 // lines first, then over the elements) fall in different banks.  Measured
 // at 256^3 against one element: FP32 y pass 38.4 -> 37.0 us, mixed step
 // 1.576 -> 1.561 ms (four elements: the same; eight: the z pass slower)
-#ifndef MIF_LP_EXTRA
-#define MIF_LP_EXTRA 2
-#endif
-constexpr int lds_pitch(int L) { return L + L / 8 + MIF_LP_EXTRA; }
+// (Two, four and six extra elements, the latter two conflict-free in a
+// 64-bank model of the load and store phases, measured within 1 % of each
+// other on the 256^3 y and z passes, FP64 and FP32.)
+constexpr int lds_pitch(int L) { return L + L / 8 + 2; }
 
 // Twiddle entries the stages read: stage S reads tw[k * L / (Ns R)] for
-// k < Ns, all below L / R, so the table needs L / (smallest radix) entries
-// (with MIF_TW_POW; the direct form reads up to L).
-constexpr int tw_count(int L) { return MIF_TW_POW ? L / min_radix(L) : L; }
+// k < Ns, all below L / R, so the table needs L / (smallest radix) entries.
+constexpr int tw_count(int L) { return L / min_radix(L); }
 
 template <typename T, int L, int NP, int SIGN, int S>
 __device__ inline void stage(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
@@ -147,7 +142,6 @@ __device__ inline void stage(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
 #pragma unroll
       for (int r = 0; r < R; r++) v[b][r] = Z[p][pd(jb + r * m)];
       if (Ns > 1) {
-#if MIF_TW_POW
         // one twiddle from the table, the others as its powers (products at
         // most three deep: w2 = w1 w1, w4 = w2 w2, w3 = w1 w2, w5 = w1 w4,
         // w6 = w2 w4, w7 = w3 w4), one LDS read per butterfly instead of R-1
@@ -164,14 +158,6 @@ __device__ inline void stage(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
         }
 #pragma unroll
         for (int r = 1; r < R; r++) v[b][r] = mul(v[b][r], w[r]);
-#else
-#pragma unroll
-        for (int r = 1; r < R; r++) {
-          Cx<T> w = tw[r * k * tstep];
-          if (SIGN > 0) w.im = -w.im;
-          v[b][r] = mul(v[b][r], w);
-        }
-#endif
       }
     }
   }
@@ -203,8 +189,6 @@ __device__ inline void fft(Cx<T> (*Z)[lds_pitch(L)], const Cx<T>* tw) {
 using p2::Cx;
 using p2::pd;
 
-// LDS bytes one workgroup may hold (gfx950: 160 KB per CU, all of it
-// available to one workgroup)
 // Stores and loads of the transform passes on the packed buffer stay
 // ordinary: the next pass reads the buffer back from the Infinity Cache
 // (non-temporal stores there took the 256^3 FP32 y pass from 39 to 43 us,
@@ -216,22 +200,12 @@ using p2::pd;
 // 1249 us, the projection after it 3578 -> 3427 us).  A run-time choice
 // here is not kept: the compiler merges the two store branches into plain
 // stores.
-#ifndef MIF_UNPACK_NT
-#define MIF_UNPACK_NT 1
-#endif
-#if MIF_UNPACK_NT
 #define MIF_ST_U(addr, val) __builtin_nontemporal_store((val), (addr))
-#else
-#define MIF_ST_U(addr, val) (*(addr) = (val))
-#endif
 
-#ifndef MIF_LDS_MAX
-#define MIF_LDS_MAX 65536
-#endif
-
-#ifndef MIF_TWL_MAX
-#define MIF_TWL_MAX 65536
-#endif
+// LDS bytes of one transform workgroup (the static __shared__ limit of a
+// kernel), and the bound under which the twiddle table joins the lines in LDS
+constexpr int kLdsMax = 65536;
+constexpr int kTwlMax = 65536;
 
 // MODE 0: forward, 1: inverse, 2: forward + scale + inverse (axis 2 only).
 // FUSED (axis 0 only): MODE 0 reads div(u)/dt from the velocity instead of
@@ -246,12 +220,12 @@ template <typename T, int L, int NP, int AXIS, int MODE, bool FUSED, typename TV
 __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W, SpectralScale sc, T* __restrict__ B,
                                                 long bp0, long bplane, int periodic, FusedIO io) {
   static_assert(!DC || (FUSED && AXIS == 0), "defect-correction outputs belong to the fused x passes");
-  // twiddles live in LDS when the workgroup stays within MIF_TWL_MAX
+  // twiddles live in LDS when the workgroup stays within kTwlMax
   // bytes, otherwise they are read from the (L1-resident) global table
   constexpr int LP = p2::lds_pitch(L);
   constexpr int TWN = p2::tw_count(L);
-  constexpr bool TWL = (NP * LP + TWN) * sizeof(Cx<T>) <= MIF_TWL_MAX;
-  static_assert(NP * LP * sizeof(Cx<T>) + (L / 2 + 1) * sizeof(double) <= MIF_LDS_MAX - (TWL ? TWN * sizeof(Cx<T>) : 0),
+  constexpr bool TWL = (NP * LP + TWN) * sizeof(Cx<T>) <= kTwlMax;
+  static_assert(NP * LP * sizeof(Cx<T>) + (L / 2 + 1) * sizeof(double) <= kLdsMax - (TWL ? TWN * sizeof(Cx<T>) : 0),
                 "workgroup LDS above the limit");
   __shared__ Cx<T> Z[NP][LP];
   __shared__ Cx<T> tws[TWL ? TWN : 1];
@@ -618,7 +592,7 @@ __global__ void __launch_bounds__(256) k_dct1_y(DGrid g, const T* __restrict__ W
   constexpr int SEG = NH / TPL;
   static_assert(N % 2 == 0 && NH % TPL == 0, "scan segments");
   constexpr int TWN = p2::tw_count(N);
-  static_assert((NP * LP + TWN + NH + 1) * sizeof(Cx<T>) <= MIF_LDS_MAX, "workgroup LDS above the limit");
+  static_assert((NP * LP + TWN + NH + 1) * sizeof(Cx<T>) <= kLdsMax, "workgroup LDS above the limit");
   constexpr int NI = NP * (NH + 1);         // pre/post items: (pair, j <= N/2)
   constexpr int PER = (NI + 255) / 256;
   __shared__ Cx<T> Z[NP][LP];
@@ -791,7 +765,7 @@ __global__ void __launch_bounds__(256) k_dct2_y(DGrid g, const T* __restrict__ W
   constexpr int STEP = 256 / NL;
   constexpr int MQ = (N + STEP - 1) / STEP;
   constexpr int TWN = p2::tw_count(N);
-  static_assert((NP * LP + TWN + N) * sizeof(Cx<T>) <= MIF_LDS_MAX, "workgroup LDS above the limit");
+  static_assert((NP * LP + TWN + N) * sizeof(Cx<T>) <= kLdsMax, "workgroup LDS above the limit");
   constexpr int NI = NP * N;  // post/pre items: (pair, k)
   constexpr int PER = (NI + 255) / 256;
   __shared__ Cx<T> Z[NP][LP];
@@ -940,122 +914,33 @@ template bool launch_dct2_y_any<float>(const DGrid&, const FftPlan&, int, float*
 // per workgroup give more workgroups in flight.  The x pass reads whole
 // rows, so it takes the smallest NP that keeps 256 threads busy.
 // (Chosen by measurement on the 256^3 and 128^3 channels.)
-#ifndef MIF_NP_D64
-#define MIF_NP_D64 16
-#endif
-#ifndef MIF_NP_D128
-#define MIF_NP_D128 8
-#endif
-#ifndef MIF_NP_D256
-#define MIF_NP_D256 4
-#endif
-#ifndef MIF_NP_D512
-#define MIF_NP_D512 2
-#endif
-#ifndef MIF_NP_F64
-#define MIF_NP_F64 16
-#endif
-#ifndef MIF_NP_F128
-#define MIF_NP_F128 16
-#endif
-#ifndef MIF_NP_F256
-#define MIF_NP_F256 8
-#endif
-#ifndef MIF_NP_F512
-#define MIF_NP_F512 4
-#endif
-// x pass
-#ifndef MIF_NPX_D256
-#define MIF_NPX_D256 2
-#endif
-#ifndef MIF_NPX_D512
-#define MIF_NPX_D512 2
-#endif
-#ifndef MIF_NPX_F256
-#define MIF_NPX_F256 4
-#endif
-#ifndef MIF_NPX_F512
-#define MIF_NPX_F512 4
-#endif
-
-#ifndef MIF_NP_D768
-#define MIF_NP_D768 2
-#endif
-#ifndef MIF_NP_F768
-#define MIF_NP_F768 4
-#endif
-#ifndef MIF_NP_D1024
-#define MIF_NP_D1024 1
-#endif
-#ifndef MIF_NP_F1024
-#define MIF_NP_F1024 2
-#endif
-#ifndef MIF_NP_D1536
-#define MIF_NP_D1536 1
-#endif
-#ifndef MIF_NP_F1536
-#define MIF_NP_F1536 2
-#endif
-#ifndef MIF_NPX_D768
-#define MIF_NPX_D768 1
-#endif
-#ifndef MIF_NPX_F768
-#define MIF_NPX_F768 2
-#endif
-// lengths 3 * 2^k below 768, as their power-of-two neighbours
-#ifndef MIF_NP_D96
-#define MIF_NP_D96 8
-#endif
-#ifndef MIF_NP_F96
-#define MIF_NP_F96 16
-#endif
-#ifndef MIF_NP_D192
-#define MIF_NP_D192 4
-#endif
-#ifndef MIF_NP_F192
-#define MIF_NP_F192 8
-#endif
-#ifndef MIF_NPX_D192
-#define MIF_NPX_D192 2
-#endif
-#ifndef MIF_NPX_F192
-#define MIF_NPX_F192 4
-#endif
-#ifndef MIF_NP_D384
-#define MIF_NP_D384 2
-#endif
-#ifndef MIF_NP_F384
-#define MIF_NP_F384 4
-#endif
-#ifndef MIF_NPX_D384
-#define MIF_NPX_D384 2
-#endif
-#ifndef MIF_NPX_F384
-#define MIF_NPX_F384 4
-#endif
-
-// Line pairs per workgroup of the half-length Neumann y pass.
-#ifndef MIF_DCT_NP_F512
-#define MIF_DCT_NP_F512 4
-#endif
-#ifndef MIF_DCT_NP_D512
-#define MIF_DCT_NP_D512 2
-#endif
-#ifndef MIF_DCT_NP_F768
-#define MIF_DCT_NP_F768 4
-#endif
-#ifndef MIF_DCT_NP_D768
-#define MIF_DCT_NP_D768 2
-#endif
-#ifndef MIF_DCT_NP_F1024
-#define MIF_DCT_NP_F1024 2
-#endif
-#ifndef MIF_DCT_NP_D1024
-#define MIF_DCT_NP_D1024 1
-#endif
-#ifndef MIF_DCT_HALF
-#define MIF_DCT_HALF 1
-#endif
+constexpr int np_strided(bool d, int L) {
+  switch (L) {
+    case 64: return 16;
+    case 128: return d ? 8 : 16;
+    case 256: return d ? 4 : 8;
+    case 512: return d ? 2 : 4;
+    case 768: return d ? 2 : 4;
+    case 1024: return d ? 1 : 2;
+    case 1536: return d ? 1 : 2;
+    // lengths 3 * 2^k below 768, as their power-of-two neighbours
+    case 96: return d ? 8 : 16;
+    case 192: return d ? 4 : 8;
+    case 384: return d ? 2 : 4;
+    default: return 1;
+  }
+}
+// x pass (lengths from 192 on; below, one kernel serves all three passes)
+constexpr int np_x(bool d, int L) {
+  switch (L) {
+    case 192: case 256: case 384: return d ? 2 : 4;
+    case 512: return d ? 2 : 4;
+    case 768: return d ? 1 : 2;
+    default: return np_strided(d, L);
+  }
+}
+// the half-length Neumann y pass (k_dct1_y, N = L / 2)
+constexpr int np_dct1(bool d, int N) { return N >= 1024 ? (d ? 1 : 2) : (d ? 2 : 4); }
 
 template <typename T, int N, int NP>
 static void launch_dct1_y(const DGrid& g, const FftPlan& plan, T* B, long bp0, long bplane, hipStream_t st) {
@@ -1074,13 +959,13 @@ static bool launch_dct1_y_any(const DGrid& g, const FftPlan& plan, T* B, long bp
   constexpr bool d = sizeof(T) == 8;
   switch (plan.L / 2) {
     case 512:
-      launch_dct1_y<T, 512, (d ? MIF_DCT_NP_D512 : MIF_DCT_NP_F512)>(g, plan, B, bp0, bplane, st);
+      launch_dct1_y<T, 512, np_dct1(d, 512)>(g, plan, B, bp0, bplane, st);
       return true;
     case 768:
-      launch_dct1_y<T, 768, (d ? MIF_DCT_NP_D768 : MIF_DCT_NP_F768)>(g, plan, B, bp0, bplane, st);
+      launch_dct1_y<T, 768, np_dct1(d, 768)>(g, plan, B, bp0, bplane, st);
       return true;
     case 1024:
-      launch_dct1_y<T, 1024, (d ? MIF_DCT_NP_D1024 : MIF_DCT_NP_F1024)>(g, plan, B, bp0, bplane, st);
+      launch_dct1_y<T, 1024, np_dct1(d, 1024)>(g, plan, B, bp0, bplane, st);
       return true;
     default:
       return false;
@@ -1157,60 +1042,60 @@ bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, co
                      long bplane, const FusedIO* io, hipStream_t st) {
   constexpr bool d = sizeof(T) == 8;
   // Neumann y: both directions are the same DCT-I
-  if (MIF_DCT_HALF && axis == 1 && !plan.periodic && mode != 2 &&
+  if (axis == 1 && !plan.periodic && mode != 2 &&
       launch_dct1_y_any<T>(g, plan, B, bp0, bplane, st))
     return true;
   switch (plan.L) {
     case 64:
-      launch_p2<T, 64, (d ? MIF_NP_D64 : MIF_NP_F64)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      launch_p2<T, 64, np_strided(d, 64)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 128:
-      launch_p2<T, 128, (d ? MIF_NP_D128 : MIF_NP_F128)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      launch_p2<T, 128, np_strided(d, 128)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 256:
       if (axis == 0)
-        launch_p2<T, 256, (d ? MIF_NPX_D256 : MIF_NPX_F256), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 256, np_x(d, 256), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
-        launch_p2<T, 256, (d ? MIF_NP_D256 : MIF_NP_F256), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 256, np_strided(d, 256), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 512:
       if (axis == 0)
-        launch_p2<T, 512, (d ? MIF_NPX_D512 : MIF_NPX_F512), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 512, np_x(d, 512), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
-        launch_p2<T, 512, (d ? MIF_NP_D512 : MIF_NP_F512), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 512, np_strided(d, 512), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 1024:
       if (axis == 0)
         launch_p2<T, 1024, 1, 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
-        launch_p2<T, 1024, (d ? MIF_NP_D1024 : MIF_NP_F1024), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 1024, np_strided(d, 1024), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 96:
-      launch_p2<T, 96, (d ? MIF_NP_D96 : MIF_NP_F96)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      launch_p2<T, 96, np_strided(d, 96)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 192:
       if (axis == 0)
-        launch_p2<T, 192, (d ? MIF_NPX_D192 : MIF_NPX_F192), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 192, np_x(d, 192), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
-        launch_p2<T, 192, (d ? MIF_NP_D192 : MIF_NP_F192), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 192, np_strided(d, 192), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 384:
       if (axis == 0)
-        launch_p2<T, 384, (d ? MIF_NPX_D384 : MIF_NPX_F384), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 384, np_x(d, 384), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
-        launch_p2<T, 384, (d ? MIF_NP_D384 : MIF_NP_F384), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 384, np_strided(d, 384), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 768:
       if (axis == 0)
-        launch_p2<T, 768, (d ? MIF_NPX_D768 : MIF_NPX_F768), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 768, np_x(d, 768), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
-        launch_p2<T, 768, (d ? MIF_NP_D768 : MIF_NP_F768), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 768, np_strided(d, 768), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 1536:
       if (axis == 0)
         launch_p2<T, 1536, 1, 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
-        launch_p2<T, 1536, (d ? MIF_NP_D1536 : MIF_NP_F1536), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+        launch_p2<T, 1536, np_strided(d, 1536), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     default:
       return false;
@@ -1222,14 +1107,14 @@ bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, co
 int fused_x_blocks(const DGrid& g, const FftPlan& plan, bool fp64) {
   int np;
   switch (plan.L) {
-    case 64: np = fp64 ? MIF_NP_D64 : MIF_NP_F64; break;
-    case 128: np = fp64 ? MIF_NP_D128 : MIF_NP_F128; break;
-    case 96: np = fp64 ? MIF_NP_D96 : MIF_NP_F96; break;
-    case 256: np = fp64 ? MIF_NPX_D256 : MIF_NPX_F256; break;
-    case 512: np = fp64 ? MIF_NPX_D512 : MIF_NPX_F512; break;
-    case 192: np = fp64 ? MIF_NPX_D192 : MIF_NPX_F192; break;
-    case 384: np = fp64 ? MIF_NPX_D384 : MIF_NPX_F384; break;
-    case 768: np = fp64 ? MIF_NPX_D768 : MIF_NPX_F768; break;
+    case 64: np = np_strided(fp64, 64); break;
+    case 128: np = np_strided(fp64, 128); break;
+    case 96: np = np_strided(fp64, 96); break;
+    case 256: np = np_x(fp64, 256); break;
+    case 512: np = np_x(fp64, 512); break;
+    case 192: np = np_x(fp64, 192); break;
+    case 384: np = np_x(fp64, 384); break;
+    case 768: np = np_x(fp64, 768); break;
     case 1024: case 1536: np = 1; break;
     default: return 0;
   }
