@@ -28,7 +28,10 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic words per pressure unknown per launch of each kernel class:
 # each logical array read once and written once per kernel (stencil
 # neighbour re-reads and ghost planes counted once; O(N^2) boundary work
-# excluded).  Stage launches alternate 10, 13, 10 words (mean 11).
+# excluded).  Stage launches alternate 10, 13, 10 words (mean 11); in the
+# FP64 form, whose stages 2 and 3 carry the previous stage's projection
+# (phi read, pressure written), 10, 15, 12 (mean 37/3), and one projection
+# launch per step remains.
 WORDS_PER_LAUNCH = {
     "stage": 11.0,
     "div_rhs": 4.0,
@@ -53,9 +56,14 @@ BYTES_FP64_TRANSFORMS = {
 }
 
 
+FUSED_STAGE_WORDS = 37.0 / 3.0
+
+
 def bytes_per_unknown(name: str, res) -> float:
     if res["esize"] == 4 and res.get("solve_transforms") == "fp64" and name in BYTES_FP64_TRANSFORMS:
         return BYTES_FP64_TRANSFORMS[name]
+    if name == "stage" and res.get("fused_projection"):
+        return FUSED_STAGE_WORDS * res["esize"]
     return WORDS_PER_LAUNCH[name] * res["esize"]
 
 
@@ -135,6 +143,9 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, 
         "block_step_s": blocks,
         "unknowns": n * (n + 1) * n,
         "esize": 8 if precision == mif.FP64 else 4,
+        # the library fuses the projections of stages 1 and 2 into the next
+        # stage kernel in the FP64 form
+        "fused_projection": precision == mif.FP64,
         "kernels": kernels,
         "velocity": final,
         "pressure": pres,
@@ -405,8 +416,9 @@ def roofline_of(res):
 
 
 # arrays read and written per launch of a kernel class (one entry per launch
-# kind): the stage kinds 1, 2, 3 and the projection
-STREAM_MIX = {"stage": [(4, 6), (7, 6), (7, 3)], "project": [(5, 4)]}
+# kind): the stage kinds 1, 2, 3 (unfused, and with the projection fused into
+# kinds 2 and 3) and the projection
+STREAM_MIX = {"stage": [(4, 6), (7, 6), (7, 3)], "stage_fused": [(4, 6), (8, 7), (8, 4)], "project": [(5, 4)]}
 
 
 def stream_ceiling(cls: str, res, roof):
@@ -418,6 +430,8 @@ def stream_ceiling(cls: str, res, roof):
 
     if cls not in STREAM_MIX:
         return None
+    if cls == "stage" and res.get("fused_projection"):
+        cls = "stage_fused"
     nbytes = res["unknowns"] * res["esize"]
     ctx = mif.Context(channel_grid(8, 8, 8), precision=mif.FP64, device=0)
     us = [ctx.stream_probe(r, w, nbytes, reps=10) for r, w in STREAM_MIX[cls]]

@@ -247,10 +247,20 @@ int mif_pressure_gradient(const mif_field *pressure, mif_field *const out[3]);
  * Constant body force f added to the momentum right-hand side N of every
  * stage of mif_timestep (default 0, which is the reference's equation). */
 int mif_ctx_set_body_force(mif_ctx *ctx, const double force[3]);
+/* keep != 0: mif_timestep (and mif_timestep_nhn) leave Y2 and Y3 after their
+ * projections in the `buf` and `buf2` triples, the stage records
+ * mif_timestep_adjoint reads.  By default (keep = 0) the projections of
+ * stages 1 and 2 are fused into the next stage's kernel, which reads the
+ * buffers before their projection, and the buffers then hold Y2 and Y3
+ * without it; u^{n+1} and p^{n+1} are the same either way (to rounding).
+ * No counterpart in the reference, whose timestep leaves its buffers
+ * unspecified to callers. */
+int mif_ctx_set_stage_records(mif_ctx *ctx, int keep);
 /* Adjoint of one mif_timestep (homogeneous pressure-increment conditions).
  * un, y2, y3 are the velocity fields the forward step used as stencil
  * sources: u^n, and the `buf` and `buf2` triples as mif_timestep leaves them
- * (Y2 and Y3 after their projections).  On entry ubar and pbar hold the
+ * with stage records on (Y2 and Y3 after their projections, see
+ * mif_ctx_set_stage_records).  On entry ubar and pbar hold the
  * adjoints of (u^{n+1}, p^{n+1}); on return those of (u^n, p^n).
  * force_bar[3] is incremented by the adjoint of the body force, which
  * synchronises the stream; with force_bar == NULL it is added to an
@@ -299,7 +309,9 @@ typedef enum mif_kernel_class {
   MIF_KC_FFT_X_DIV = 9, /* divergence fused into the x forward pass        */
   MIF_KC_FFT_X_UNPACK = 10, /* x inverse pass writing the pressure field   */
   MIF_KC_ADJOINT = 11,  /* whole adjoint step (all its kernels)            */
-  MIF_KC_COUNT = 12
+  MIF_KC_PRESSURE_RIM = 12, /* p += phi on the points a stage kernel with a fused
+                               projection leaves out                        */
+  MIF_KC_COUNT = 13
 } mif_kernel_class;
 int mif_ctx_set_timing(mif_ctx *ctx, int enable);
 /* Total milliseconds and launch count of a class since timing was enabled

@@ -102,6 +102,12 @@ struct mif_ctx {
   // the increment the last solve stored still holds its weighted mean
   // (red_phi); the next projection subtracts it from the pressure update
   const Red3* phi_mean = nullptr;
+  // timestep: the projections of stages 1 and 2 are fused into the next
+  // stage kernel unless the caller asked for the stage records (the
+  // projected Y2, Y3 in the step's buffers, which mif_timestep_adjoint reads);
+  // pswap is the second pressure buffer the fused stages write into
+  bool stage_records = false;
+  void* pswap = nullptr;
   double* B64 = nullptr;   // mixed form: FP64 right-hand side
   double* P64 = nullptr;   // mixed form: FP64 accumulated solution
   double* R64 = nullptr;   // stand-alone Poisson runs: stored FP64 right-hand side
@@ -600,9 +606,13 @@ int finish_standalone_solve(mif_ctx* c, mif_field* p) {
 int pressure_solve_any(mif_ctx* c, mif_field* p, mif_field* const vel[3], double dt, const FnDesc& gp, double tg,
                        int nhn);
 
+// phi (not null): the previous stage's projection fused in, with time step
+// pdts and the increment's mean c->phi_mean; the incremented pressure goes to
+// pout, here and (launch_p_rim) on the points the stage's tiles leave out.
 template <typename T>
 int stage(mif_ctx* c, int st, mif_field* const X[3], mif_field* const A[3], mif_field* const O1[3],
-          mif_field* const O2[3], mif_field* p, double a1, double a2, double b) {
+          mif_field* const O2[3], const void* p, double a1, double a2, double b, const void* phi = nullptr,
+          double pdts = 0, void* pout = nullptr) {
   StageArgs s{};
   for (int q = 0; q < 3; q++) {
     s.X[q] = X[q]->data;
@@ -610,13 +620,24 @@ int stage(mif_ctx* c, int st, mif_field* const X[3], mif_field* const A[3], mif_
     s.O1[q] = O1[q]->data;
     s.O2[q] = O2 ? O2[q]->data : nullptr;
   }
-  s.p = p->data;
+  s.p = p;
   s.a1 = a1;
   s.a2 = a2;
   s.b = b;
   for (int q = 0; q < 3; q++) s.force[q] = c->force[q];
-  KTimer t(c, MIF_KC_STAGE);
-  launch_stage<T>(st, c->g, c->L, s, c->stream);
+  s.phi = phi;
+  s.pdts = pdts;
+  s.pmean = phi ? c->phi_mean : nullptr;
+  s.pout = pout;
+  {
+    KTimer t(c, MIF_KC_STAGE);
+    launch_stage<T>(st, c->g, c->L, s, c->stream);
+  }
+  if (phi) {
+    KTimer t(c, MIF_KC_PRESSURE_RIM);
+    launch_p_rim<T>(c->g, c->L, (const T*)p, (const T*)phi, (T*)pout, c->phi_mean, c->stream);
+    c->phi_mean = nullptr;
+  }
   return MIF_OK;
 }
 
@@ -668,26 +689,44 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   const int nhn = gd != nullptr;
   const FnDesc none{};
   int rc;
+  // Fused form (FP64): the projections of stages 1 and 2 (velocity
+  // correction and pressure increment) are applied by the stage kernels of
+  // stages 2 and 3 as they read Y2, Y3 and p (one pass over the velocity and
+  // pressure fewer per fused projection).  The pressure increments alternate
+  // between p and the context's second pressure buffer and end in p after
+  // stage 3.  (The FP32 stage kernels are not fused: launch_stage.)
+  const bool fuse = !c->stage_records && sizeof(T) == 8;
+  if (fuse && !c->pswap) HIPCHK(c, alloc_field_storage(c, &c->pswap));
+  void* const pd = p->data;
+  void* const ps = c->pswap;
   // Stage 1 (src/Timestep.cpp:107-117)
   {
     const double a1 = 64.0 / 120.0 * dt;
-    if ((rc = stage<T>(c, 1, vel, nullptr, buf, buf2, p, a1, 0.0, a1))) return rc;
+    if ((rc = stage<T>(c, 1, vel, nullptr, buf, buf2, pd, a1, 0.0, a1))) return rc;
   }
   apply_bc<T>(c, buf, ev[0], st.time[1]);
   if ((rc = pressure_solve_any(c, pb, buf, st.dt_s[1], nhn ? gd[0] : none, 0.0, nhn))) return rc;
-  project<T>(c, buf, p, pb, st.dt_s[1]);
+  if (!fuse) project<T>(c, buf, p, pb, st.dt_s[1]);
   // Stage 2 (src/Timestep.cpp:119-129)
   {
     const double a1 = -34.0 / 120.0 * dt, a2 = 50.0 / 120.0 * dt;
-    if ((rc = stage<T>(c, 2, buf, buf2, buf2, vel, p, a1, a2, a1 + a2))) return rc;
+    if (fuse)
+      rc = stage<T>(c, 2, buf, buf2, buf2, vel, pd, a1, a2, a1 + a2, pb->data, st.dt_s[1], ps);
+    else
+      rc = stage<T>(c, 2, buf, buf2, buf2, vel, pd, a1, a2, a1 + a2);
+    if (rc) return rc;
   }
   apply_bc<T>(c, buf2, ev[1], st.time[2]);
   if ((rc = pressure_solve_any(c, pb, buf2, st.dt_s[2], nhn ? gd[1] : none, 0.0, nhn))) return rc;
-  project<T>(c, buf2, p, pb, st.dt_s[2]);
+  if (!fuse) project<T>(c, buf2, p, pb, st.dt_s[2]);
   // Stage 3 (src/Timestep.cpp:131-140)
   {
     const double a2 = -50.0 / 120.0 * dt, a3 = 90.0 / 120.0 * dt;
-    if ((rc = stage<T>(c, 3, buf2, vel, vel, nullptr, p, 0.0, a3, a2 + a3))) return rc;
+    if (fuse)
+      rc = stage<T>(c, 3, buf2, vel, vel, nullptr, ps, 0.0, a3, a2 + a3, pb->data, st.dt_s[2], pd);
+    else
+      rc = stage<T>(c, 3, buf2, vel, vel, nullptr, pd, 0.0, a3, a2 + a3);
+    if (rc) return rc;
   }
   apply_bc<T>(c, vel, ev[2], st.time[3]);
   if ((rc = pressure_solve_any(c, pb, vel, st.dt_s[3], nhn ? gd[2] : none, 0.0, nhn))) return rc;
@@ -1193,6 +1232,7 @@ int mif_ctx_destroy(mif_ctx* c) {
     f->ctx = nullptr;
   }
   c->fields.clear();
+  free_field_storage(c, c->pswap);
   (void)hipFree(c->B);
   (void)hipFree(c->B64);
   (void)hipFree(c->P64);
@@ -1232,6 +1272,12 @@ int mif_ctx_info(const mif_ctx* c, double* dt, uint64_t ext[3]) {
   if (dt) *dt = c->g.dt;
   if (ext)
     for (int a = 0; a < 3; a++) ext[a] = (uint64_t)c->g.N[a];
+  return MIF_OK;
+}
+
+int mif_ctx_set_stage_records(mif_ctx* c, int keep) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  c->stage_records = keep != 0;
   return MIF_OK;
 }
 

@@ -9,6 +9,7 @@ namespace mif {
 // (src/Timestep.cpp:10-54).  X is the stencil source, A the pointwise
 // operand, O1/O2 the outputs; a1, a2, b are the stage coefficients (stage 3
 // passes its a3 in a2).
+struct Red3;
 struct StageArgs {
   const void* X[3];
   const void* A[3];
@@ -17,6 +18,16 @@ struct StageArgs {
   const void* p;
   double a1, a2, b;
   double force[3];  // constant body force added to N (zero in the reference)
+  // The previous stage's projection, fused into this stage (kinds 2 and 3):
+  // when phi is set, X holds that stage's output before its projection and p
+  // the pressure before its increment; the kernel reads them as
+  // X - pdts grad(phi) (component interiors) and p + phi - mean (mean from
+  // pmean, or 0), and stores the incremented p at its tile's points into pout
+  // (not p, which neighbouring threads and workgroups read).
+  const void* phi;
+  double pdts;
+  const Red3* pmean;
+  void* pout;
 };
 
 // Arguments of the velocity-only stage kernel (src/TimestepVelocity.cpp:20-51).
@@ -86,6 +97,11 @@ struct FusedIO {
 
 template <typename T>
 void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st);
+// pout = pin + (phi - mean) on the pressure points a stage kernel with a fused
+// projection leaves out (outside the union of the components' interiors)
+template <typename T>
+void launch_p_rim(const DGrid& g, const Layout& L, const T* pin, const T* phi, T* pout, const Red3* mean,
+                  hipStream_t st);
 template <typename T>
 void launch_vel_stage(int stage, const DGrid& g, const Layout& L, const VelStageArgs& s, hipStream_t st);
 // trig[((a * 2 + s) * 2 + q) * tn + i] = sin (q = 0) or cos (q = 1) of the

@@ -128,9 +128,17 @@ struct StageTile {
 // 16 176.5 us, 24 178.0 us, 32 192.5 us.  FP64 on 64 x 8 tiles, with the
 // plane order of k_stage's AHEAD form: 4 planes 329 us, 6 328, 8 320, 10 320,
 // 12 323, 16 336, 24 357, 32 342 (four ring slots instead of three: 8 planes
-// 318 us, 16 330 us, so three slots stay).
-template <typename T>
-constexpr int stage_planes() { return sizeof(T) == 8 ? 8 : 16; }
+// 318 us, 16 330 us, so three slots stay).  Fields beyond the Infinity Cache
+// (no non-temporal path, NT false) keep 16 planes in the full step: 512^3
+// FP64 stage kernel 2763 us at 16 planes, 2831 us at 8.  The velocity-only
+// kinds (4..6) take 8 in FP64 (256^3 step 1.53 -> 1.46 ms with the request
+// order above).
+// The FP64 kernels with a fused projection (128 VGPRs: two workgroups per CU
+// whatever the ring) take four ring slots and one barrier per plane: 256^3
+// step 2.741 -> 2.706 ms; 8 or 16 planes measured the same.
+template <typename T, int KIND, bool NT, bool PROJ = false>
+constexpr int stage_planes() { return sizeof(T) == 8 && (NT || KIND >= 4 || PROJ) ? 8 : 16; }
+constexpr int kStageSlotsFused64 = 4;
 
 // ring slots of the FP32 and FP64 stage kernels (see k_stage)
 constexpr int kStageSlots32 = 4, kStageSlots64 = 3;
@@ -141,14 +149,23 @@ constexpr bool kStageVec = true;
 // FT (velocity-only step): MIF_FN_VT_FORCING read from the trig tables.
 // VEC: halo planes fetched with 16-byte loads (element 1 of every row
 // 16-byte aligned: the layout's storage shift).
-template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = false>
-// (FP64 full step: at most 80 VGPRs, i.e. three 512-thread workgroups per CU)
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && KIND <= 3 ? 6 : 1)))
+// PROJ (kinds 2 and 3): the previous stage's projection fused in (StageArgs
+// phi): each halo element of X is corrected by -pdts grad(phi) as it is
+// committed to LDS (k_project's expression, on the component's interior),
+// the pressure gradient uses p + phi - mean, and that incremented pressure is
+// stored into pout at the tile's points (another buffer than p: neighbouring
+// threads and tiles read p).
+template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = false, bool PROJ = false>
+// (FP64 full step: at most 80 VGPRs, i.e. three 512-thread workgroups per CU;
+// with the fused projection 128, two workgroups)
+__global__ void __launch_bounds__(512)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && KIND <= 3 ? (PROJ ? 4 : 6) : 1)))
 k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int kmax, int ntx, int nty, int ntiles) {
+  static_assert(!PROJ || KIND == 2 || KIND == 3, "a fused projection belongs to stages 2 and 3");
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 and
   // more (plane k+2 is committed into a free slot while plane k is
   // computed; one barrier per plane)
-  constexpr int NS = sizeof(T) == 4 ? kStageSlots32 : kStageSlots64;
+  constexpr int NS = PROJ && sizeof(T) == 8 ? kStageSlotsFused64 : (sizeof(T) == 4 ? kStageSlots32 : kStageSlots64);
   // p at plane k-1 carried in a register from plane to plane instead of
   // loaded (FP64: 256^3 stage kernel 348 -> 341 us, step 2.955 -> 2.928 ms;
   // FP32: 181 -> 183 us on 64 x 8 tiles, 176 -> 179 us on 128 x 4, so FP32
@@ -163,7 +180,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * STX + tx;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
-  constexpr int SKB = stage_planes<T>();
+  constexpr int SKB = stage_planes<T, KIND, NT, PROJ>();
   const int i0 = bx * STX + 1, j0 = by * STY + 1, k0 = bz * SKB + 1;
   const int k1 = min(k0 + SKB, kmax + 1);
   const int i = i0 + tx, j = j0 + ty;
@@ -226,6 +243,101 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       dst[r] = q < PLANE_ELEMS ? q : -1;
     }
   }
+  // fused projection: per load group its field and the offset of its first
+  // point in the common layout (any field's), per element whether its (i, j)
+  // lies in that component's interior
+  constexpr int NG = VEC ? QV + 1 : PQ;
+  constexpr int PE = PROJ ? PR : 1;
+  int gf[NG], goff[NG];
+  bool pin[PE];
+  if constexpr (PROJ) {
+    auto in_ij = [&](int f, int gi, int gj) {
+      int e[3];
+      extents(g, f, e);
+      return gi >= 1 && gj >= 1 && gi <= e[0] - 2 && gj <= e[1] - 2;
+    };
+    if constexpr (VEC) {
+#pragma unroll
+      for (int r = 0; r < QV; r++) {
+        const int q = tid + STN * r;
+        const int f = q / (SHY * RQ), rr = q - f * (SHY * RQ);
+        const int jj = rr / RQ, iq = rr - jj * RQ;
+        const int gi = i0 + VW * iq, gj = j0 - 1 + jj;
+        // the field of a 16-byte group is uniform over each wave (a wave
+        // covers 64 consecutive q, and a field's groups are a multiple of 64)
+        gf[r] = f;
+        goff[r] = ok[r] ? (int)L.idx(gi, gj, 0) : 1;
+#pragma unroll
+        for (int e = 0; e < VW; e++) pin[r * VW + e] = ok[r] && in_ij(f, gi + e, gj);
+      }
+      const int f = tid / (SHY * 2), rr = tid - f * (SHY * 2);
+      const int jj = rr >> 1, side = rr & 1;
+      const int gi = side ? i0 + STX : i0 - 1, gj = j0 - 1 + jj;
+      gf[QV] = ok[QV] ? f : 0;
+      goff[QV] = ok[QV] ? (int)L.idx(gi, gj, 0) : 0;
+      pin[QV * VW] = ok[QV] && in_ij(f, gi, gj);
+    } else {
+#pragma unroll
+      for (int r = 0; r < PQ; r++) {
+        const int q = tid + STN * r;
+        const int f = q / (SHY * SHX), rr = q - f * (SHY * SHX);
+        const int jj = rr / SHX, ii = rr - jj * SHX;
+        const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+        gf[r] = ok[r] ? f : 0;
+        goff[r] = ok[r] ? (int)L.idx(gi, gj, 0) : 0;
+        pin[r] = ok[r] && in_ij(f, gi, gj);
+      }
+    }
+  }
+  // phi at each halo element and at its lower neighbour along its component's
+  // axis (indices clamped into the array: an element whose neighbour falls
+  // outside is no interior point, and its values are not used)
+  const T* phi = (const T*)s.phi;
+  const long plast = L.elems - 1;
+  static_assert(!VEC || ((SHY * RQ) % 64 == 0 && STN % 64 == 0), "wave-uniform fields of the 16-byte groups");
+  auto fetch_phi = [&](int k, T (&a)[PE], T (&b)[PE]) {
+    if constexpr (PROJ) {
+#pragma unroll
+      for (int r = 0; r < NG; r++) {
+        constexpr int GW0 = VEC ? VW : 1;
+        const long base = goff[r] + (long)k * L.plane;
+        if (VEC && r < QV) {
+          // 16-byte groups: phi over the group with one load at the group's
+          // (aligned) offset; below it along x from the element before and
+          // the group's own values, along y or z with one load a row or a
+          // plane back (aligned as well; an offset before the array's start
+          // belongs to no interior element and reads element 1 instead)
+          const int fu = __builtin_amdgcn_readfirstlane(gf[r]);
+          const Vec va = *(GV*)((GT*)phi + base);
+#pragma unroll
+          for (int e = 0; e < VW; e++) a[r * GW0 + e] = va[e];
+          // along x the group's predecessors are the last element of the
+          // 16-byte group before it and the group's own first elements; the
+          // storage shift leaves element -1 inside the allocation
+          const long bb = base - (fu == 0 ? VW : (fu == 1 ? (long)L.PX : L.plane));
+          const Vec vb = *(GV*)((GT*)phi + (bb >= -1 ? bb : 1L));
+#pragma unroll
+          for (int e = 0; e < VW; e++) b[r * GW0 + e] = fu == 0 ? (e == 0 ? vb[VW - 1] : va[e - 1]) : vb[e];
+        } else {
+          const long dl = gf[r] == 0 ? 1 : (gf[r] == 1 ? (long)L.PX : L.plane);
+          const int el = r * GW0;
+          a[el] = phi[min(base, plast)];
+          b[el] = phi[max(base - dl, 0L)];
+        }
+      }
+    }
+  };
+  // element el of the plane being committed, its projected value
+  auto projected = [&](int el, int r, int k, T x, const T (&a)[PE], const T (&b)[PE]) -> T {
+    if constexpr (PROJ) {
+      const int f = gf[r];
+      const int ez = f == 2 ? g.N[2] + 1 : g.N[2];
+      const T oh = (T)(f == 0 ? g.one_over_h[0] : (f == 1 ? g.one_over_h[1] : g.one_over_h[2]));
+      const T xp = x - (a[el] - b[el]) * oh * (T)s.pdts;
+      return pin[el] && k >= 1 && k <= ez - 2 ? xp : x;
+    }
+    return x;
+  };
   // every plane read lies in 0 .. kmax + 1 <= SZ - 1, so the load address is
   // valid for every lane and the loads need no branch.  Lanes outside the
   // field are zeroed when the values are committed to LDS (AHEAD: so that no
@@ -249,39 +361,53 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       }
     }
   };
-  auto commit = [&](int slot, const T (&v)[PR]) {
-    T* base = &pl[slot][0][0][0];
+  auto commit = [&](int k, const T (&v)[PR], const T (&pa)[PE], const T (&pb)[PE]) {
+    T* base = &pl[k % NS][0][0][0];
     if constexpr (VEC) {
 #pragma unroll
       for (int r = 0; r < QV; r++)
         if (dst[r] >= 0)
 #pragma unroll
-          for (int e = 0; e < VW; e++) base[dst[r] + e] = AHEAD ? mask(ok[r], v[r * VW + e]) : v[r * VW + e];
-      if (dst[QV] >= 0) base[dst[QV]] = AHEAD ? mask(ok[QV], v[QV * VW]) : v[QV * VW];
+          for (int e = 0; e < VW; e++)
+            base[dst[r] + e] = projected(r * VW + e, r, k, AHEAD ? mask(ok[r], v[r * VW + e]) : v[r * VW + e], pa, pb);
+      if (dst[QV] >= 0) base[dst[QV]] = projected(QV * VW, QV, k, AHEAD ? mask(ok[QV], v[QV * VW]) : v[QV * VW], pa, pb);
     } else {
 #pragma unroll
       for (int r = 0; r < PQ; r++)
-        if (dst[r] >= 0) base[dst[r]] = AHEAD ? mask(ok[r], v[r]) : v[r];
+        if (dst[r] >= 0) base[dst[r]] = projected(r, r, k, AHEAD ? mask(ok[r], v[r]) : v[r], pa, pb);
     }
   };
   {
     T a[PR], b[PR], c[PR];
+    T fa[3][PE], fb[3][PE];
     fetch(k0 - 1, a);
+    fetch_phi(k0 - 1, fa[0], fb[0]);
     fetch(k0, b);
+    fetch_phi(k0, fa[1], fb[1]);
     fetch(k0 + 1, c);
-    commit((k0 - 1) % NS, a);
-    commit(k0 % NS, b);
-    commit((k0 + 1) % NS, c);
+    fetch_phi(k0 + 1, fa[2], fb[2]);
+    commit(k0 - 1, a, fa[0], fb[0]);
+    commit(k0, b, fa[1], fb[1]);
+    commit(k0 + 1, c, fa[2], fb[2]);
   }
   T pre[PR];
-  if (!AHEAD && k0 + 2 <= k1) fetch(k0 + 2, pre);
+  T pfa[PE], pfb[PE];
+  if (!AHEAD && k0 + 2 <= k1) {
+    fetch(k0 + 2, pre);
+    fetch_phi(k0 + 2, pfa, pfb);
+  }
   __syncthreads();
 
   const bool in_tile = i <= imax && j <= jmax;
 
   const long doff[3] = {1, L.PX, L.plane};
+  const T mv = PROJ && s.pmean ? (T)(s.pmean->s1 / s.pmean->s2) : (T)0;
   T p_prev = (T)0;
-  if (KIND <= 3 && PCARRY && in_tile) p_prev = ((const T*)s.p)[L.idx(i, j, k0 - 1)];
+  if (KIND <= 3 && PCARRY && in_tile) {
+    const long ob = L.idx(i, j, k0 - 1);
+    p_prev = ((const T*)s.p)[ob];
+    if constexpr (PROJ) p_prev = p_prev + (phi[ob] - mv);
+  }
   // Order of the memory requests per plane k.
   // AHEAD: the pointwise operands of plane k first, then the halo plane k+2
   // into registers; the stencil work of plane k waits for the pointwise
@@ -299,12 +425,16 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   for (int k = k0; k < k1; k++) {
     if constexpr (!AHEAD && NS >= 4) {
       // the slot of plane k-2, which no thread reads any more
-      if (k + 2 <= k1) commit((k + 2) % NS, pre);
-      if (k + 3 <= k1) fetch(k + 3, pre);
+      if (k + 2 <= k1) commit(k + 2, pre, pfa, pfb);
+      if (k + 3 <= k1) {
+        fetch(k + 3, pre);
+        fetch_phi(k + 3, pfa, pfb);
+      }
     }
     const long o = L.idx(i, j, k);
     const long ol = AHEAD ? oc + (long)k * L.plane : o;
     T p0 = (T)0, pm[3] = {(T)0, (T)0, (T)0}, av[3] = {(T)0, (T)0, (T)0}, bv[3] = {(T)0, (T)0, (T)0};
+    T f0 = (T)0, fm[3] = {(T)0, (T)0, (T)0};  // fused projection: phi at the point and below it
     // pointwise operands of all three components, loaded together
     auto load_pointwise = [&](const bool (&inside)[3]) {
       if constexpr (KIND <= 3) {
@@ -315,6 +445,15 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
           if (inside[c]) pm[c] = p[-doff[c]];
         if constexpr (!PCARRY)
           if (inside[2]) pm[2] = p[-doff[2]];
+        if constexpr (PROJ) {
+          const T* ph = phi + ol;
+          f0 = ph[0];
+#pragma unroll
+          for (int c = 0; c < 2; c++)
+            if (inside[c]) fm[c] = ph[-doff[c]];
+          if constexpr (!PCARRY)
+            if (inside[2]) fm[2] = ph[-doff[2]];
+        }
         if constexpr (KIND >= 2) {
 #pragma unroll
           for (int c = 0; c < 3; c++)
@@ -336,7 +475,10 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
     // plane k+2 is read by plane k+1's stencil; past the chunk's last plane
     // the fetch repeats plane k1 (a cache hit) and is not committed
     const bool more = k + 2 <= k1;
-    if constexpr (AHEAD) fetch(min(k + 2, k1), pre);
+    if constexpr (AHEAD) {
+      fetch(min(k + 2, k1), pre);
+      fetch_phi(min(k + 2, k1), pfa, pfb);
+    }
     // AHEAD, the full step computes every component on every thread (the
     // taps of a thread outside the field still lie in the LDS tile; only the
     // stores are masked), so that the compiler keeps the loads above
@@ -346,11 +488,19 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       bool inside[3];
       for (int c = 0; c < 3; c++) inside[c] = interior(g, c, i, j, k);
       if constexpr (!AHEAD) load_pointwise(inside);
-      // p at plane k-1 of this column: the previous plane's p0 (loaded
-      // before the loop for the first plane)
+      // fused projection: the pressure after the previous stage's increment,
+      // p + (phi - mean) (k_project's expression), at the point and below it
+      T pc = p0;
+      if constexpr (PROJ) {
+        pc = p0 + (f0 - mv);
+#pragma unroll
+        for (int c = 0; c < (PCARRY ? 2 : 3); c++) pm[c] = pm[c] + (fm[c] - mv);
+      }
+      // p at plane k-1 of this column: the previous plane's (loaded before
+      // the loop for the first plane)
       if constexpr (KIND <= 3 && PCARRY) {
         pm[2] = p_prev;
-        p_prev = p0;
+        p_prev = pc;
       }
       const int slot[3] = {(k + NS - 1) % NS, k % NS, (k + 1) % NS};
       auto tap = [&](int f, int di, int dj, int dk) -> T { return pl[slot[dk + 1]][f][ty + 1 + dj][tx + 1 + di]; };
@@ -362,7 +512,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
         if constexpr (KIND <= 3) {
           rhs += (T)s.force[c];
           // pressure_gradient_{u,v,w} (include/PressureGradient.h:9-25)
-          const T pg = (p0 - pm[c]) * (T)g.one_over_h[c];
+          const T pg = (pc - pm[c]) * (T)g.one_over_h[c];
           T o1, o2 = rhs;
           if constexpr (KIND == 1) {
             o1 = x + (T)s.a1 * rhs - (T)s.b * pg;
@@ -411,20 +561,29 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
           }
         }
       }
+      if constexpr (PROJ) {
+        // threads outside the tile store into the padding column i = N0 of
+        // the pressure, beyond its extents
+        const long op = EVERY && !in_tile ? L.idx(g.N[0], 0, k) : o;
+        store_out<NT>(&((T*)s.pout)[op], pc);
+      }
     }
     // plane k+2 goes to the slot of plane k-1 (three slots: once every
     // thread is done with plane k) or of plane k-2 (four and more slots: no
     // thread has read it since the previous barrier)
     if constexpr (AHEAD) {
       if constexpr (NS == 3) __syncthreads();
-      if (more) commit((k + 2) % NS, pre);
+      if (more) commit(k + 2, pre, pfa, pfb);
       __syncthreads();
     } else {
       __syncthreads();
       if constexpr (NS == 3) {
         // plane k+2 replaces plane k-1; plane k+3 starts loading
-        if (more) commit((k + 2) % NS, pre);
-        if (k + 3 <= k1) fetch(k + 3, pre);
+        if (more) commit(k + 2, pre, pfa, pfb);
+        if (k + 3 <= k1) {
+          fetch(k + 3, pre);
+          fetch_phi(k + 3, pfa, pfb);
+        }
         __syncthreads();
       }
     }
@@ -943,9 +1102,19 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
   const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
   constexpr int STX = StageTile<T>::STX, STY = StageTile<T>::STY;
-  constexpr int SKB = stage_planes<T>();
-  const int ntx = (imax + STX - 1) / STX, nty = (jmax + STY - 1) / STY, ntz = (kmax + SKB - 1) / SKB;
-  const int ntiles = ntx * nty * ntz;
+  const int ntx = (imax + STX - 1) / STX, nty = (jmax + STY - 1) / STY;
+  const dim3 blk(STX, STY, 1);
+  auto go = [&](auto vec, auto proj) {
+    constexpr bool V = decltype(vec)::value, P = decltype(proj)::value;
+    constexpr int SKB = stage_planes<T, KIND, NT, P>();
+    const int ntiles = ntx * nty * ((kmax + SKB - 1) / SKB);
+    k_stage<T, KIND, FT, NT, V, P><<<ntiles, blk, 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
+  };
+  using yes = std::true_type;
+  using no = std::false_type;
+  // the projection is fused in the FP64 kernels only: in FP32 the fused
+  // kinds took 256^3 steps from 1.50 to 1.75 ms (registers: 120-129 VGPRs)
+  constexpr bool kFuse = sizeof(T) == 8 && (KIND == 2 || KIND == 3);
   // the 16-byte loads need element 1 of every row on a 16-byte boundary:
   // the storage shift puts it on a 128-byte one, and rows are 64-byte
   // multiples.  Measured against 4- and 8-byte loads (step times): 256^3
@@ -957,12 +1126,79 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   // loads (on the FP32 128 x 4 tiles: 176.2 -> 175.9 us, no gain either).
   if constexpr (kStageVec && KIND <= 3 && (sizeof(T) == 8 || !NT)) {
     if (L.lead * (long)sizeof(T) == 128 - (long)sizeof(T) && (L.PX * sizeof(T)) % 16 == 0) {
-      k_stage<T, KIND, FT, NT, true>
-          <<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
-      return;
+      if constexpr (kFuse) {
+        if (s.phi) return go(yes{}, yes{});
+      }
+      return go(yes{}, no{});
     }
   }
-  k_stage<T, KIND, FT, NT><<<ntiles, dim3(STX, STY, 1), 0, st>>>(g, L, s, vs, imax, jmax, kmax, ntx, nty, ntiles);
+  if constexpr (kFuse) {
+    if (s.phi) return go(no{}, yes{});
+  }
+  go(no{}, no{});
+}
+
+// ---------------------------------------------------------------------------
+// The pressure increment of a projection fused into the next stage
+// (update_pressure, src/Timestep.cpp:76-81: p += phi at every point), on the
+// points the stage kernel's tiles leave out: those of the pressure's extents
+// outside [1, imax] x [1, jmax] x [1, kmax] (imax, jmax, kmax: the union of
+// the velocity components' interiors).  pout = pin + (phi - mean).
+// Three groups of points: the k planes outside [1, kmax]; within it the j
+// rows outside [1, jmax]; within both the i columns outside [1, imax].
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int rim_out_count(int n, int mx) { return 1 + max(0, n - 1 - mx); }
+__device__ inline int rim_out_index(int t, int mx) { return t == 0 ? 0 : mx + t; }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_p_rim(DGrid g, Layout L, const T* __restrict__ pin, const T* __restrict__ phi,
+                                               T* __restrict__ pout, const Red3* __restrict__ mean, int imax, int jmax,
+                                               int kmax) {
+  const int grp = blockIdx.y;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n0 = g.N[0], n1 = g.N[1], n2 = g.N[2];
+  const int ki = min(kmax, n2 - 1), ji = min(jmax, n1 - 1);  // inner ranges [1, ki], [1, ji]
+  int i, j, k;
+  if (grp == 0) {
+    if (q >= (long)rim_out_count(n2, kmax) * n0 * n1) return;
+    i = (int)(q % n0);
+    j = (int)((q / n0) % n1);
+    k = rim_out_index((int)(q / ((long)n0 * n1)), kmax);
+  } else if (grp == 1) {
+    if (q >= (long)rim_out_count(n1, jmax) * n0 * ki) return;
+    i = (int)(q % n0);
+    const long r = q / n0;
+    j = rim_out_index((int)(r % rim_out_count(n1, jmax)), jmax);
+    k = 1 + (int)(r / rim_out_count(n1, jmax));
+  } else {
+    const int no = rim_out_count(n0, imax);
+    if (q >= (long)no * ji * ki) return;
+    i = rim_out_index((int)(q % no), imax);
+    const long r = q / no;
+    j = 1 + (int)(r % ji);
+    k = 1 + (int)(r / ji);
+  }
+  const T mv = mean ? (T)(mean->s1 / mean->s2) : (T)0;
+  const long o = L.idx(i, j, k);
+  pout[o] = pin[o] + (phi[o] - mv);
+}
+
+template <typename T>
+void launch_p_rim(const DGrid& g, const Layout& L, const T* pin, const T* phi, T* pout, const Red3* mean,
+                  hipStream_t st) {
+  int eu[3], ev[3], ew[3];
+  extents(g, 0, eu);
+  extents(g, 1, ev);
+  extents(g, 2, ew);
+  const int imax = max(eu[0], max(ev[0], ew[0])) - 2;
+  const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
+  const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
+  const int n0 = g.N[0], n1 = g.N[1], n2 = g.N[2];
+  const long c0 = (long)rim_out_count(n2, kmax) * n0 * n1;
+  const long c1 = (long)rim_out_count(n1, jmax) * n0 * min(kmax, n2 - 1);
+  const long c2 = (long)rim_out_count(n0, imax) * min(jmax, n1 - 1) * min(kmax, n2 - 1);
+  const long mx = max(c0, max(c1, c2));
+  k_p_rim<T><<<dim3((unsigned)((mx + 255) / 256), 3, 1), 256, 0, st>>>(g, L, pin, phi, pout, mean, imax, jmax, kmax);
 }
 
 template <typename T>
@@ -1160,6 +1396,7 @@ void launch_gather(const Layout& L, const T* F, const int* ijk, long n, double* 
 
 #define MIF_INSTANTIATE(T)                                                                                          \
   template void launch_stage<T>(int, const DGrid&, const Layout&, const StageArgs&, hipStream_t);                   \
+  template void launch_p_rim<T>(const DGrid&, const Layout&, const T*, const T*, T*, const Red3*, hipStream_t);    \
   template void launch_vel_stage<T>(int, const DGrid&, const Layout&, const VelStageArgs&, hipStream_t);            \
   template void launch_bc<T>(const DGrid&, const Layout&, void* const*, const FnDesc&, double, hipStream_t);         \
   template void launch_periodic<T>(const DGrid&, const Layout&, T*, int, hipStream_t);                              \

@@ -44,7 +44,7 @@ FN_TIME_DIFFERENCE = 12
 
 # mif_kernel_class
 KERNEL_CLASSES = ["stage", "bc", "div_rhs", "fft_x", "fft_y", "fft_z_fused", "unpack", "project", "defect",
-                  "fft_x_div", "fft_x_unpack", "adjoint"]
+                  "fft_x_div", "fft_x_unpack", "adjoint", "p_rim"]
 
 
 class MifGrid(ctypes.Structure):
@@ -112,6 +112,7 @@ SIGNATURES = {
     "mif_error_norms_pressure_tabulated": [_VP, _VP, _DP],
     "mif_pressure_gradient": [_VP, _V3],
     "mif_ctx_set_body_force": [_VP, _DP],
+    "mif_ctx_set_stage_records": [_VP, _I],
     "mif_timestep_adjoint": [_V3, _V3, _V3, _V3, _VP, _DP],
     "mif_adjoint_force_take": [_VP, _DP],
     "mif_field_copy": [_VP, _VP],

@@ -34,6 +34,9 @@ class Rollout:
         if ctx.precision != L.FP64 and ctx.precision != L.MIXED:
             raise ValueError("unknown precision")
         self.ctx = ctx
+        # the adjoint steps read the projected stage states the forward
+        # leaves in its buffers
+        ctx.set_stage_records(True)
         self.nsteps = int(nsteps)
         self.k = max(1, int(checkpoint_every))
         self.ev = exact_velocity if exact_velocity is not None else function(L.FN_ZERO)

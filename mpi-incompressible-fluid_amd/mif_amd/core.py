@@ -146,6 +146,12 @@ class Context:
         self.check(self.lib.mif_ctx_get_solve_precision(self.ptr, ctypes.byref(eff), ctypes.byref(kappa)))
         return eff.value, kappa.value
 
+    def set_stage_records(self, keep: bool):
+        """keep: timestep() leaves the projected stage states Y2, Y3 in its
+        two buffers (what timestep_adjoint reads); otherwise the projections
+        of stages 1 and 2 are fused into the next stage kernel (the default)."""
+        self.check(self.lib.mif_ctx_set_stage_records(self.ptr, 1 if keep else 0))
+
     def set_body_force(self, f):
         """Constant body force added to N in every stage of timestep()
         (zero = the reference's equation)."""

@@ -53,6 +53,7 @@ def _oracle_adjoint(o, u0, p0, f, wu, wp, nsteps):
 def _gpu_adjoint(gs, u0, p0, f, wu, wp, nsteps, device_force=False):
     ctx = mif.Context(gs)
     ctx.set_body_force(f)
+    ctx.set_stage_records(True)
     vel, buf, buf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
     p, pb = ctx.pressure(), ctx.pressure()
     vel.upload([u0[c] for c in range(3)])
