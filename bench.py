@@ -77,7 +77,8 @@ def block_sizes(steps: int):
     return [steps // r + (1 if i < steps % r else 0) for i in range(r)]
 
 
-def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, keep_fields: bool = True):
+def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, keep_fields: bool = True,
+            schedule: int = 0):
     import mif_amd as mif
     from mif_amd.problems import channel_grid
 
@@ -85,6 +86,7 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, 
     ctx = mif.Context(gs, precision=precision, device=0)
     if precision == mif.MIXED:
         ctx.set_defect_sweeps(sweeps)
+        ctx.set_defect_schedule(schedule)
     vel, buf, buf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
     p, pb = ctx.pressure(), ctx.pressure()
     vel.upload(init)
@@ -562,14 +564,17 @@ def main():
 
     init = channel_initial_velocity(channel_grid(args.n, args.n, args.n))
     forms = {}
-    runs = {"fp64": (mif.FP64, 0)}
+    # (precision, defect sweeps, defect schedule); mixed_s1_last sweeps in
+    # the step's last pressure solve only
+    runs = {"fp64": (mif.FP64, 0, 0)}
     if not args.fp64_only:
-        runs["mixed_s0"] = (mif.MIXED, 0)
-        runs["mixed_s1"] = (mif.MIXED, 1)
+        runs["mixed_s0"] = (mif.MIXED, 0, mif.DEFECT_EVERY_STAGE)
+        runs["mixed_s1"] = (mif.MIXED, 1, mif.DEFECT_EVERY_STAGE)
+        runs["mixed_s1_last"] = (mif.MIXED, 1, mif.DEFECT_LAST_STAGE)
     results = {}
-    for name, (prec, sweeps) in runs.items():
+    for name, (prec, sweeps, sched) in runs.items():
         print(f"[bench] {name}: {args.warmup} + {args.steps} steps at {args.n}^3", file=sys.stderr, flush=True)
-        results[name] = run_gpu(args.n, args.steps, args.warmup, prec, sweeps, init)
+        results[name] = run_gpu(args.n, args.steps, args.warmup, prec, sweeps, init, schedule=sched)
     ref = results["fp64"]
     for name, r in results.items():
         rate = r["unknowns"] / r["step_s"]
@@ -596,7 +601,7 @@ def main():
     # relative velocity error of FP64 ("equal error") and its speed-up.
     chosen = "fp64"
     best_mixed = None
-    for name in ("mixed_s0", "mixed_s1"):
+    for name in ("mixed_s0", "mixed_s1", "mixed_s1_last"):
         f = forms.get(name)
         if f and f["finite"] and f["velocity_rel_l2_vs_fp64"] <= 1e-6:
             if best_mixed is None or f["cell_updates_per_s"] > forms[best_mixed]["cell_updates_per_s"]:

@@ -78,6 +78,19 @@ int mif_ctx_info(const mif_ctx *ctx, double *dt, uint64_t node_extents[3]);
 int mif_ctx_set_defect_sweeps(mif_ctx *ctx, int sweeps);
 /* Defect-correction sweep count of the context. */
 int mif_ctx_get_defect_sweeps(const mif_ctx *ctx, int *sweeps);
+/* Which pressure solves of mif_timestep and mif_timestep_nhn (and their
+ * _tabulated forms) take the defect sweeps (no reference counterpart; the reference solves in one
+ * precision).  MIF_DEFECT_EVERY_STAGE (the default): all three.
+ * MIF_DEFECT_LAST_STAGE: the solve of stage 3 only, whose increment
+ * completes p^{n+1} and whose projection leaves u^{n+1}; the solves of
+ * stages 1 and 2 use the FP32 transforms alone.  Stand-alone solves
+ * (mif_solve_pressure_*, mif_poisson_solve, mif_poisson_run) always take
+ * the sweeps. */
+typedef enum mif_defect_schedule {
+  MIF_DEFECT_EVERY_STAGE = 0,
+  MIF_DEFECT_LAST_STAGE = 1
+} mif_defect_schedule;
+int mif_ctx_set_defect_schedule(mif_ctx *ctx, int schedule);
 
 /* Transforms of the mixed form's pressure solve (no reference counterpart;
  * the reference solves in one precision).  FP32 transforms amplify their

@@ -107,6 +107,7 @@ struct mif_ctx {
   // projected Y2, Y3 in the step's buffers, which mif_timestep_adjoint reads);
   // pswap is the second pressure buffer the fused stages write into
   bool stage_records = false;
+  int defect_schedule = MIF_DEFECT_EVERY_STAGE;
   void* pswap = nullptr;
   double* B64 = nullptr;   // mixed form: FP64 right-hand side
   double* P64 = nullptr;   // mixed form: FP64 accumulated solution
@@ -699,13 +700,22 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   if (fuse && !c->pswap) HIPCHK(c, alloc_field_storage(c, &c->pswap));
   void* const pd = p->data;
   void* const ps = c->pswap;
+  // Pressure solve of stage k; with MIF_DEFECT_LAST_STAGE the mixed form's
+  // defect sweeps run in the solve of stage 3 only.
+  auto solve = [&](int k, mif_field* const* u) {
+    const int sweeps = c->sweeps;
+    if (k < 3 && c->defect_schedule == MIF_DEFECT_LAST_STAGE) c->sweeps = 0;
+    const int r = pressure_solve_any(c, pb, u, st.dt_s[k], nhn ? gd[k - 1] : none, 0.0, nhn);
+    c->sweeps = sweeps;
+    return r;
+  };
   // Stage 1 (src/Timestep.cpp:107-117)
   {
     const double a1 = 64.0 / 120.0 * dt;
     if ((rc = stage<T>(c, 1, vel, nullptr, buf, buf2, pd, a1, 0.0, a1))) return rc;
   }
   apply_bc<T>(c, buf, ev[0], st.time[1]);
-  if ((rc = pressure_solve_any(c, pb, buf, st.dt_s[1], nhn ? gd[0] : none, 0.0, nhn))) return rc;
+  if ((rc = solve(1, buf))) return rc;
   if (!fuse) project<T>(c, buf, p, pb, st.dt_s[1]);
   // Stage 2 (src/Timestep.cpp:119-129)
   {
@@ -717,7 +727,7 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
     if (rc) return rc;
   }
   apply_bc<T>(c, buf2, ev[1], st.time[2]);
-  if ((rc = pressure_solve_any(c, pb, buf2, st.dt_s[2], nhn ? gd[1] : none, 0.0, nhn))) return rc;
+  if ((rc = solve(2, buf2))) return rc;
   if (!fuse) project<T>(c, buf2, p, pb, st.dt_s[2]);
   // Stage 3 (src/Timestep.cpp:131-140)
   {
@@ -729,7 +739,7 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
     if (rc) return rc;
   }
   apply_bc<T>(c, vel, ev[2], st.time[3]);
-  if ((rc = pressure_solve_any(c, pb, vel, st.dt_s[3], nhn ? gd[2] : none, 0.0, nhn))) return rc;
+  if ((rc = solve(3, vel))) return rc;
   project<T>(c, vel, p, pb, st.dt_s[3]);
   LAUNCHCHK(c);
   return MIF_OK;
@@ -1285,6 +1295,14 @@ int mif_ctx_set_defect_sweeps(mif_ctx* c, int sweeps) {
   if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
   if (sweeps < 0 || sweeps > 4) return fail(c, MIF_ERR_INVALID_ARGUMENT, "sweeps must be in 0..4");
   c->sweeps = sweeps;
+  return MIF_OK;
+}
+
+int mif_ctx_set_defect_schedule(mif_ctx* c, int schedule) {
+  if (!c) return fail(nullptr, MIF_ERR_INVALID_ARGUMENT, "null context");
+  if (schedule != MIF_DEFECT_EVERY_STAGE && schedule != MIF_DEFECT_LAST_STAGE)
+    return fail(c, MIF_ERR_INVALID_ARGUMENT, "unknown defect schedule %d", schedule);
+  c->defect_schedule = schedule;
   return MIF_OK;
 }
 

@@ -27,6 +27,9 @@ FP64, MIXED = 0, 1
 # mif_solve_precision
 SOLVE_AUTO, SOLVE_FP32, SOLVE_FP64 = 0, 1, 2
 
+# mif_defect_schedule
+DEFECT_EVERY_STAGE, DEFECT_LAST_STAGE = 0, 1
+
 # mif_fn_kind
 FN_ZERO = 0
 FN_ES_VELOCITY = 1
@@ -82,6 +85,7 @@ SIGNATURES = {
     "mif_ctx_info": [_VP, _DP, _U64P],
     "mif_ctx_set_defect_sweeps": [_VP, _I],
     "mif_ctx_get_defect_sweeps": [_VP, ctypes.POINTER(_I)],
+    "mif_ctx_set_defect_schedule": [_VP, _I],
     "mif_ctx_set_solve_precision": [_VP, _I],
     "mif_ctx_get_solve_precision": [_VP, ctypes.POINTER(_I), _DP],
     "mif_field_create": [_VP, _I, ctypes.POINTER(_VP)],

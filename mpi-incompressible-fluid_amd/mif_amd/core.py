@@ -134,6 +134,13 @@ class Context:
     def set_defect_sweeps(self, n: int):
         self.check(self.lib.mif_ctx_set_defect_sweeps(self.ptr, n))
 
+    def set_defect_schedule(self, schedule: int):
+        """Which solves of timestep() take the defect sweeps:
+        L.DEFECT_EVERY_STAGE (all three, the default) or L.DEFECT_LAST_STAGE
+        (the solve of stage 3 only; stages 1 and 2 use FP32 transforms
+        alone).  Stand-alone solves always take them."""
+        self.check(self.lib.mif_ctx_set_defect_schedule(self.ptr, schedule))
+
     def set_solve_precision(self, mode: int):
         """Transforms of the mixed form's pressure solve: L.SOLVE_AUTO (FP32
         up to condition number 6e5, FP64 above), L.SOLVE_FP32, L.SOLVE_FP64."""
