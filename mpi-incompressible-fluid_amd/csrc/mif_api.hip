@@ -690,13 +690,12 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   const int nhn = gd != nullptr;
   const FnDesc none{};
   int rc;
-  // Fused form (FP64): the projections of stages 1 and 2 (velocity
-  // correction and pressure increment) are applied by the stage kernels of
-  // stages 2 and 3 as they read Y2, Y3 and p (one pass over the velocity and
-  // pressure fewer per fused projection).  The pressure increments alternate
-  // between p and the context's second pressure buffer and end in p after
-  // stage 3.  (The FP32 stage kernels are not fused: launch_stage.)
-  const bool fuse = !c->stage_records && sizeof(T) == 8;
+  // Fused form: the projections of stages 1 and 2 (velocity correction and
+  // pressure increment) are applied by the stage kernels of stages 2 and 3
+  // as they read Y2, Y3 and p (one pass over the velocity and pressure fewer
+  // per fused projection).  The pressure increments alternate between p and
+  // the context's second pressure buffer and end in p after stage 3.
+  const bool fuse = !c->stage_records;
   if (fuse && !c->pswap) HIPCHK(c, alloc_field_storage(c, &c->pswap));
   void* const pd = p->data;
   void* const ps = c->pswap;

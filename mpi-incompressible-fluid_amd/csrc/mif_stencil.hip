@@ -159,7 +159,7 @@ template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = fal
 // (FP64 full step: at most 80 VGPRs, i.e. three 512-thread workgroups per CU;
 // with the fused projection 128, two workgroups)
 __global__ void __launch_bounds__(512)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && KIND <= 3 ? (PROJ ? 4 : 6) : 1)))
+__attribute__((amdgpu_waves_per_eu(KIND <= 3 && (sizeof(T) == 8 || PROJ) ? (sizeof(T) == 8 && PROJ ? 4 : 6) : 1)))
 k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int kmax, int ntx, int nty, int ntiles) {
   static_assert(!PROJ || KIND == 2 || KIND == 3, "a fused projection belongs to stages 2 and 3");
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 and
@@ -173,10 +173,18 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   constexpr bool PCARRY = sizeof(T) == 8;
   // order of the memory requests per plane (see the plane loop)
   constexpr bool AHEAD = sizeof(T) == 8;
+  // fused projection with the phi planes staged in LDS (the FP32 kernel):
+  // phi is read once per plane instead of at every halo element and below it
+  constexpr bool PL = PROJ && !AHEAD;
+  static_assert(!PL || NS >= 4, "the LDS phi ring runs with one barrier per plane");
   using TT = StageTile<T>;
   constexpr int STX = TT::STX, STY = TT::STY, SHX = TT::SHX, SHY = TT::SHY, STN = TT::STN;
   constexpr int PLANE_ELEMS = TT::PLANE_ELEMS, PQ = TT::PQ;
   __shared__ T pl[NS][3][SHY][SHX];
+  // PL: phi on planes k .. k+3 over the halo plane's points plus one more row
+  // and column below (the lower neighbours along x and y)
+  constexpr int NSP = 4, PHX = SHX + 1, PHE = (SHY + 1) * PHX, PHQ = (PHE + STN - 1) / STN;
+  __shared__ T phl[PL ? NSP : 1][PL ? PHE : 1];
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * STX + tx;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
@@ -197,7 +205,11 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   constexpr int QV = (NIV + STN - 1) / STN;
   constexpr int NHV = 3 * SHY * 2;                // halo-column elements per plane
   static_assert(NHV <= STN, "one halo-column element per thread");
-  constexpr int PR = VEC ? QV * VW + 1 : PQ;      // plane values a thread holds
+  // PL: the halo plane is fetched field by field: slot s of a thread is
+  // point q = tid + STN s of the SHY x SHX plane, the same point of u, v and
+  // w, so that the three share its phi values and its index arithmetic
+  constexpr int FQ = (SHY * SHX + STN - 1) / STN;
+  constexpr int PR = PL ? 3 * FQ : (VEC ? QV * VW + 1 : PQ);  // plane values a thread holds
   typedef T Vec __attribute__((ext_vector_type(VW)));
   // AHEAD, halo loads go through global-address-space pointers: generic
   // (flat) loads also count as LDS operations, so every wait for the LDS
@@ -247,10 +259,10 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // point in the common layout (any field's), per element whether its (i, j)
   // lies in that component's interior
   constexpr int NG = VEC ? QV + 1 : PQ;
-  constexpr int PE = PROJ ? PR : 1;
+  constexpr int PE = PROJ && !PL ? PR : 1;
   int gf[NG], goff[NG];
   bool pin[PE];
-  if constexpr (PROJ) {
+  if constexpr (PROJ && !PL) {
     auto in_ij = [&](int f, int gi, int gj) {
       int e[3];
       extents(g, f, e);
@@ -294,9 +306,59 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // outside is no interior point, and its values are not used)
   const T* phi = (const T*)s.phi;
   const long plast = L.elems - 1;
+  // PL: per slot the source offset of its point (a point outside the
+  // arrays reads element 0: its value is masked), whether it is inside, its
+  // (i, j) and its index in the LDS phi plane
+  constexpr int FQS = PL ? FQ : 1;
+  int xo[FQS], xgi[FQS], xgj[FQS], xix[FQS];
+  bool xok[FQS];
+  if constexpr (PL) {
+#pragma unroll
+    for (int r = 0; r < FQ; r++) {
+      const int q = tid + STN * r;
+      const int jj = q / SHX, ii = q - jj * SHX;
+      xgi[r] = i0 - 1 + ii;
+      xgj[r] = j0 - 1 + jj;
+      xok[r] = q < SHY * SHX && xgi[r] < L.PX && xgj[r] < L.SY;
+      xo[r] = xok[r] ? (int)L.idx(xgi[r], xgj[r], 0) : 0;
+      xix[r] = (jj + 1) * PHX + ii + 1;
+    }
+  }
+  // PL: this thread's share of a phi plane, source column and LDS index tid +
+  // STN r (a column outside the array reads element 0 of the plane: it is
+  // the lower neighbour of no interior element)
+  constexpr int PQS = PL ? PHQ : 1;
+  int poff[PQS];
+  if constexpr (PL) {
+#pragma unroll
+    for (int r = 0; r < PHQ; r++) {
+      const int q = tid + STN * r;
+      const int jj = q / PHX, ii = q - jj * PHX;
+      const int gi = i0 - 2 + ii, gj = j0 - 2 + jj;
+      const bool okp = q < PHE && gi >= 0 && gj >= 0 && gi < L.PX && gj < L.SY;
+      poff[r] = okp ? (int)L.idx(gi, gj, 0) : 0;
+    }
+  }
+  // PL: phi plane k (plane 0 for k = -1, which feeds no interior element)
+  // into registers, and from there into its LDS slot
+  auto pfetch = [&](int k, T (&v)[PQS]) {
+    if constexpr (PL) {
+      const T* pk = phi + (long)max(k, 0) * L.plane;
+#pragma unroll
+      for (int r = 0; r < PHQ; r++) v[r] = pk[poff[r]];
+    }
+  };
+  auto pcommit = [&](int k, const T (&v)[PQS]) {
+    if constexpr (PL) {
+      T* base = &phl[(k + NSP) % NSP][0];
+#pragma unroll
+      for (int r = 0; r < PHQ; r++)
+        if (tid + STN * r < PHE) base[tid + STN * r] = v[r];
+    }
+  };
   static_assert(!VEC || ((SHY * RQ) % 64 == 0 && STN % 64 == 0), "wave-uniform fields of the 16-byte groups");
   auto fetch_phi = [&](int k, T (&a)[PE], T (&b)[PE]) {
-    if constexpr (PROJ) {
+    if constexpr (PROJ && !PL) {
 #pragma unroll
       for (int r = 0; r < NG; r++) {
         constexpr int GW0 = VEC ? VW : 1;
@@ -329,7 +391,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   };
   // element el of the plane being committed, its projected value
   auto projected = [&](int el, int r, int k, T x, const T (&a)[PE], const T (&b)[PE]) -> T {
-    if constexpr (PROJ) {
+    if constexpr (PROJ && !PL) {
       const int f = gf[r];
       const int ez = f == 2 ? g.N[2] + 1 : g.N[2];
       const T oh = (T)(f == 0 ? g.one_over_h[0] : (f == 1 ? g.one_over_h[1] : g.one_over_h[2]));
@@ -344,7 +406,12 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // instruction waits for the loads before the commit) or as they arrive.
   auto mask = [&](bool keep, T x) { return keep ? x : (T)0; };
   auto fetch = [&](int k, T (&v)[PR]) {
-    if constexpr (VEC) {
+    if constexpr (PL) {
+#pragma unroll
+      for (int r = 0; r < FQ; r++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) v[f * FQ + r] = mask(xok[r], X[f][xo[r] + (long)k * L.plane]);
+    } else if constexpr (VEC) {
 #pragma unroll
       for (int r = 0; r < QV; r++) {
         const Vec x = *(GV*)(src[r] + (long)k * L.plane);
@@ -363,7 +430,28 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   };
   auto commit = [&](int k, const T (&v)[PR], const T (&pa)[PE], const T (&pb)[PE]) {
     T* base = &pl[k % NS][0][0][0];
-    if constexpr (VEC) {
+    if constexpr (PL) {
+      // projected values X - pdts grad(phi) on each field's interior, phi at
+      // the point and below it from the LDS planes k and k-1 (k >= 0 here)
+      const T* cur = &phl[k % NSP][0];
+      const T* prv = &phl[(k + NSP - 1) % NSP][0];
+#pragma unroll
+      for (int r = 0; r < FQ; r++) {
+        if (tid + STN * r >= SHY * SHX) continue;
+        const int ix = xix[r];
+        const T fc = cur[ix], lo[3] = {cur[ix - 1], cur[ix - PHX], prv[ix]};
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+          int e[3];
+          extents(g, f, e);
+          const bool in = xok[r] && xgi[r] >= 1 && xgj[r] >= 1 && xgi[r] <= e[0] - 2 && xgj[r] <= e[1] - 2 && k >= 1 &&
+                          k <= e[2] - 2;
+          const T x = v[f * FQ + r];
+          const T xp = x - (fc - lo[f]) * (T)g.one_over_h[f] * (T)s.pdts;
+          base[f * (SHY * SHX) + tid + STN * r] = in ? xp : x;
+        }
+      }
+    } else if constexpr (VEC) {
 #pragma unroll
       for (int r = 0; r < QV; r++)
         if (dst[r] >= 0)
@@ -377,6 +465,16 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
         if (dst[r] >= 0) base[dst[r]] = projected(r, r, k, AHEAD ? mask(ok[r], v[r]) : v[r], pa, pb);
     }
   };
+  if constexpr (PL) {
+    // phi planes k0-2 .. k0+1, which the projections of the first three
+    // planes read
+    T q[4][PQS];
+#pragma unroll
+    for (int d = 0; d < 4; d++) pfetch(k0 - 2 + d, q[d]);
+#pragma unroll
+    for (int d = 0; d < 4; d++) pcommit(k0 - 2 + d, q[d]);
+    __syncthreads();
+  }
   {
     T a[PR], b[PR], c[PR];
     T fa[3][PE], fb[3][PE];
@@ -392,9 +490,21 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   }
   T pre[PR];
   T pfa[PE], pfb[PE];
+  T pq[PQS];
   if (!AHEAD && k0 + 2 <= k1) {
     fetch(k0 + 2, pre);
     fetch_phi(k0 + 2, pfa, pfb);
+    pfetch(k0 + 2, pq);
+  }
+  // PL: phi below this thread's point on the plane before the current one
+  // (carried from plane to plane)
+  T fprev = (T)0;
+  if constexpr (PL) {
+    fprev = phl[(k0 - 1) % NSP][(ty + 2) * PHX + tx + 2];
+    // plane k0+2 takes the slot of plane k0-2 once every projection above
+    // is done
+    __syncthreads();
+    if (k0 + 2 <= k1) pcommit(k0 + 2, pq);
   }
   __syncthreads();
 
@@ -429,6 +539,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       if (k + 3 <= k1) {
         fetch(k + 3, pre);
         fetch_phi(k + 3, pfa, pfb);
+        pfetch(k + 3, pq);
       }
     }
     const long o = L.idx(i, j, k);
@@ -445,7 +556,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
           if (inside[c]) pm[c] = p[-doff[c]];
         if constexpr (!PCARRY)
           if (inside[2]) pm[2] = p[-doff[2]];
-        if constexpr (PROJ) {
+        if constexpr (PROJ && !PL) {
           const T* ph = phi + ol;
           f0 = ph[0];
 #pragma unroll
@@ -491,6 +602,15 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       // fused projection: the pressure after the previous stage's increment,
       // p + (phi - mean) (k_project's expression), at the point and below it
       T pc = p0;
+      if constexpr (PL) {
+        const T* cur = &phl[k % NSP][0];
+        const int ix = (ty + 2) * PHX + tx + 2;
+        f0 = cur[ix];
+        fm[0] = cur[ix - 1];
+        fm[1] = cur[ix - PHX];
+        fm[2] = fprev;
+        fprev = f0;
+      }
       if constexpr (PROJ) {
         pc = p0 + (f0 - mv);
 #pragma unroll
@@ -576,6 +696,10 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       if (more) commit(k + 2, pre, pfa, pfb);
       __syncthreads();
     } else {
+      // PL: phi plane k+3 into the slot of plane k-1 (read last by plane
+      // k-1's pointwise terms)
+      if constexpr (PL)
+        if (k + 3 <= k1) pcommit(k + 3, pq);
       __syncthreads();
       if constexpr (NS == 3) {
         // plane k+2 replaces plane k-1; plane k+3 starts loading
@@ -1112,9 +1236,8 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   };
   using yes = std::true_type;
   using no = std::false_type;
-  // the projection is fused in the FP64 kernels only: in FP32 the fused
-  // kinds took 256^3 steps from 1.50 to 1.75 ms (registers: 120-129 VGPRs)
-  constexpr bool kFuse = sizeof(T) == 8 && (KIND == 2 || KIND == 3);
+  // kinds 2 and 3 take the previous stage's projection when s.phi is set
+  constexpr bool kFuse = KIND == 2 || KIND == 3;
   // the 16-byte loads need element 1 of every row on a 16-byte boundary:
   // the storage shift puts it on a 128-byte one, and rows are 64-byte
   // multiples.  Measured against 4- and 8-byte loads (step times): 256^3

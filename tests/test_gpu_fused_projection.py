@@ -77,8 +77,9 @@ def test_fused_matches_unfused_mixed(sweeps):
     gs, o, init, nsteps, fn = _initial(case)
     (fv, fp) = _run(gs, o, init, nsteps, fn, records=False, precision=mif.MIXED, sweeps=sweeps)
     (uv, up) = _run(gs, o, init, nsteps, fn, records=True, precision=mif.MIXED, sweeps=sweeps)
-    # FP32 fields: the two sequences may round the same expressions apart by
-    # an ulp or two per stage (fused multiply-adds chosen per kernel)
+    # FP32 fields (the fused kernel stages phi through LDS): the two sequences
+    # may round the same expressions apart by an ulp or two per stage (fused
+    # multiply-adds chosen per kernel)
     for c in range(3):
         assert rel_err(fv[c], uv[c]) < 1e-6, c
         assert np.all(np.isfinite(fv[c]))
