@@ -133,12 +133,11 @@ struct StageTile {
 // FP64 stage kernel 2763 us at 16 planes, 2831 us at 8.  The velocity-only
 // kinds (4..6) take 8 in FP64 (256^3 step 1.53 -> 1.46 ms with the request
 // order above).
-// The FP64 kernels with a fused projection (128 VGPRs: two workgroups per CU
-// whatever the ring) take four ring slots and one barrier per plane: 256^3
-// step 2.741 -> 2.706 ms; 8 or 16 planes measured the same.
+// The FP64 kernels with a fused projection take three ring slots: with the
+// LDS phi ring (71 KB of LDS, two workgroups per CU) a fourth would leave one.
 template <typename T, int KIND, bool NT, bool PROJ = false>
 constexpr int stage_planes() { return sizeof(T) == 8 && (NT || KIND >= 4 || PROJ) ? 8 : 16; }
-constexpr int kStageSlotsFused64 = 4;
+constexpr int kStageSlotsFused64 = 3;
 
 // ring slots of the FP32 and FP64 stage kernels (see k_stage)
 constexpr int kStageSlots32 = 4, kStageSlots64 = 3;
@@ -157,7 +156,8 @@ constexpr bool kStageVec = true;
 // threads and tiles read p).
 template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = false, bool PROJ = false>
 // (FP64 full step: at most 80 VGPRs, i.e. three 512-thread workgroups per CU;
-// with the fused projection 128, two workgroups)
+// with the fused projection 128, two workgroups, which its LDS allows anyway;
+// FP32 with the fused projection: at most 80 VGPRs, three workgroups)
 __global__ void __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(KIND <= 3 && (sizeof(T) == 8 || PROJ) ? (sizeof(T) == 8 && PROJ ? 4 : 6) : 1)))
 k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int kmax, int ntx, int nty, int ntiles) {
@@ -173,10 +173,10 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   constexpr bool PCARRY = sizeof(T) == 8;
   // order of the memory requests per plane (see the plane loop)
   constexpr bool AHEAD = sizeof(T) == 8;
-  // fused projection with the phi planes staged in LDS (the FP32 kernel):
+  // fused projection: the phi planes are staged through an LDS ring, so that
   // phi is read once per plane instead of at every halo element and below it
-  constexpr bool PL = PROJ && !AHEAD;
-  static_assert(!PL || NS >= 4, "the LDS phi ring runs with one barrier per plane");
+  constexpr bool PL = PROJ;
+  static_assert(!PL || AHEAD || NS >= 4, "the FP32 order runs the phi ring with one barrier per plane");
   using TT = StageTile<T>;
   constexpr int STX = TT::STX, STY = TT::STY, SHX = TT::SHX, SHY = TT::SHY, STN = TT::STN;
   constexpr int PLANE_ELEMS = TT::PLANE_ELEMS, PQ = TT::PQ;
@@ -255,57 +255,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       dst[r] = q < PLANE_ELEMS ? q : -1;
     }
   }
-  // fused projection: per load group its field and the offset of its first
-  // point in the common layout (any field's), per element whether its (i, j)
-  // lies in that component's interior
-  constexpr int NG = VEC ? QV + 1 : PQ;
-  constexpr int PE = PROJ && !PL ? PR : 1;
-  int gf[NG], goff[NG];
-  bool pin[PE];
-  if constexpr (PROJ && !PL) {
-    auto in_ij = [&](int f, int gi, int gj) {
-      int e[3];
-      extents(g, f, e);
-      return gi >= 1 && gj >= 1 && gi <= e[0] - 2 && gj <= e[1] - 2;
-    };
-    if constexpr (VEC) {
-#pragma unroll
-      for (int r = 0; r < QV; r++) {
-        const int q = tid + STN * r;
-        const int f = q / (SHY * RQ), rr = q - f * (SHY * RQ);
-        const int jj = rr / RQ, iq = rr - jj * RQ;
-        const int gi = i0 + VW * iq, gj = j0 - 1 + jj;
-        // the field of a 16-byte group is uniform over each wave (a wave
-        // covers 64 consecutive q, and a field's groups are a multiple of 64)
-        gf[r] = f;
-        goff[r] = ok[r] ? (int)L.idx(gi, gj, 0) : 1;
-#pragma unroll
-        for (int e = 0; e < VW; e++) pin[r * VW + e] = ok[r] && in_ij(f, gi + e, gj);
-      }
-      const int f = tid / (SHY * 2), rr = tid - f * (SHY * 2);
-      const int jj = rr >> 1, side = rr & 1;
-      const int gi = side ? i0 + STX : i0 - 1, gj = j0 - 1 + jj;
-      gf[QV] = ok[QV] ? f : 0;
-      goff[QV] = ok[QV] ? (int)L.idx(gi, gj, 0) : 0;
-      pin[QV * VW] = ok[QV] && in_ij(f, gi, gj);
-    } else {
-#pragma unroll
-      for (int r = 0; r < PQ; r++) {
-        const int q = tid + STN * r;
-        const int f = q / (SHY * SHX), rr = q - f * (SHY * SHX);
-        const int jj = rr / SHX, ii = rr - jj * SHX;
-        const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
-        gf[r] = ok[r] ? f : 0;
-        goff[r] = ok[r] ? (int)L.idx(gi, gj, 0) : 0;
-        pin[r] = ok[r] && in_ij(f, gi, gj);
-      }
-    }
-  }
-  // phi at each halo element and at its lower neighbour along its component's
-  // axis (indices clamped into the array: an element whose neighbour falls
-  // outside is no interior point, and its values are not used)
   const T* phi = (const T*)s.phi;
-  const long plast = L.elems - 1;
   // PL: per slot the source offset of its point (a point outside the
   // arrays reads element 0: its value is masked), whether it is inside, its
   // (i, j) and its index in the LDS phi plane
@@ -343,7 +293,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // into registers, and from there into its LDS slot
   auto pfetch = [&](int k, T (&v)[PQS]) {
     if constexpr (PL) {
-      const T* pk = phi + (long)max(k, 0) * L.plane;
+      GT* pk = (GT*)phi + (long)max(k, 0) * L.plane;
 #pragma unroll
       for (int r = 0; r < PHQ; r++) v[r] = pk[poff[r]];
     }
@@ -356,50 +306,6 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
         if (tid + STN * r < PHE) base[tid + STN * r] = v[r];
     }
   };
-  static_assert(!VEC || ((SHY * RQ) % 64 == 0 && STN % 64 == 0), "wave-uniform fields of the 16-byte groups");
-  auto fetch_phi = [&](int k, T (&a)[PE], T (&b)[PE]) {
-    if constexpr (PROJ && !PL) {
-#pragma unroll
-      for (int r = 0; r < NG; r++) {
-        constexpr int GW0 = VEC ? VW : 1;
-        const long base = goff[r] + (long)k * L.plane;
-        if (VEC && r < QV) {
-          // 16-byte groups: phi over the group with one load at the group's
-          // (aligned) offset; below it along x from the element before and
-          // the group's own values, along y or z with one load a row or a
-          // plane back (aligned as well; an offset before the array's start
-          // belongs to no interior element and reads element 1 instead)
-          const int fu = __builtin_amdgcn_readfirstlane(gf[r]);
-          const Vec va = *(GV*)((GT*)phi + base);
-#pragma unroll
-          for (int e = 0; e < VW; e++) a[r * GW0 + e] = va[e];
-          // along x the group's predecessors are the last element of the
-          // 16-byte group before it and the group's own first elements; the
-          // storage shift leaves element -1 inside the allocation
-          const long bb = base - (fu == 0 ? VW : (fu == 1 ? (long)L.PX : L.plane));
-          const Vec vb = *(GV*)((GT*)phi + (bb >= -1 ? bb : 1L));
-#pragma unroll
-          for (int e = 0; e < VW; e++) b[r * GW0 + e] = fu == 0 ? (e == 0 ? vb[VW - 1] : va[e - 1]) : vb[e];
-        } else {
-          const long dl = gf[r] == 0 ? 1 : (gf[r] == 1 ? (long)L.PX : L.plane);
-          const int el = r * GW0;
-          a[el] = phi[min(base, plast)];
-          b[el] = phi[max(base - dl, 0L)];
-        }
-      }
-    }
-  };
-  // element el of the plane being committed, its projected value
-  auto projected = [&](int el, int r, int k, T x, const T (&a)[PE], const T (&b)[PE]) -> T {
-    if constexpr (PROJ && !PL) {
-      const int f = gf[r];
-      const int ez = f == 2 ? g.N[2] + 1 : g.N[2];
-      const T oh = (T)(f == 0 ? g.one_over_h[0] : (f == 1 ? g.one_over_h[1] : g.one_over_h[2]));
-      const T xp = x - (a[el] - b[el]) * oh * (T)s.pdts;
-      return pin[el] && k >= 1 && k <= ez - 2 ? xp : x;
-    }
-    return x;
-  };
   // every plane read lies in 0 .. kmax + 1 <= SZ - 1, so the load address is
   // valid for every lane and the loads need no branch.  Lanes outside the
   // field are zeroed when the values are committed to LDS (AHEAD: so that no
@@ -410,7 +316,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
 #pragma unroll
       for (int r = 0; r < FQ; r++)
 #pragma unroll
-        for (int f = 0; f < 3; f++) v[f * FQ + r] = mask(xok[r], X[f][xo[r] + (long)k * L.plane]);
+        for (int f = 0; f < 3; f++) v[f * FQ + r] = ((GT*)X[f])[xo[r] + (long)k * L.plane];
     } else if constexpr (VEC) {
 #pragma unroll
       for (int r = 0; r < QV; r++) {
@@ -428,7 +334,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       }
     }
   };
-  auto commit = [&](int k, const T (&v)[PR], const T (&pa)[PE], const T (&pb)[PE]) {
+  auto commit = [&](int k, const T (&v)[PR]) {
     T* base = &pl[k % NS][0][0][0];
     if constexpr (PL) {
       // projected values X - pdts grad(phi) on each field's interior, phi at
@@ -446,7 +352,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
           extents(g, f, e);
           const bool in = xok[r] && xgi[r] >= 1 && xgj[r] >= 1 && xgi[r] <= e[0] - 2 && xgj[r] <= e[1] - 2 && k >= 1 &&
                           k <= e[2] - 2;
-          const T x = v[f * FQ + r];
+          const T x = mask(xok[r], v[f * FQ + r]);
           const T xp = x - (fc - lo[f]) * (T)g.one_over_h[f] * (T)s.pdts;
           base[f * (SHY * SHX) + tid + STN * r] = in ? xp : x;
         }
@@ -457,12 +363,12 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
         if (dst[r] >= 0)
 #pragma unroll
           for (int e = 0; e < VW; e++)
-            base[dst[r] + e] = projected(r * VW + e, r, k, AHEAD ? mask(ok[r], v[r * VW + e]) : v[r * VW + e], pa, pb);
-      if (dst[QV] >= 0) base[dst[QV]] = projected(QV * VW, QV, k, AHEAD ? mask(ok[QV], v[QV * VW]) : v[QV * VW], pa, pb);
+            base[dst[r] + e] = AHEAD ? mask(ok[r], v[r * VW + e]) : v[r * VW + e];
+      if (dst[QV] >= 0) base[dst[QV]] = AHEAD ? mask(ok[QV], v[QV * VW]) : v[QV * VW];
     } else {
 #pragma unroll
       for (int r = 0; r < PQ; r++)
-        if (dst[r] >= 0) base[dst[r]] = projected(r, r, k, AHEAD ? mask(ok[r], v[r]) : v[r], pa, pb);
+        if (dst[r] >= 0) base[dst[r]] = AHEAD ? mask(ok[r], v[r]) : v[r];
     }
   };
   if constexpr (PL) {
@@ -477,25 +383,17 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   }
   {
     T a[PR], b[PR], c[PR];
-    T fa[3][PE], fb[3][PE];
     fetch(k0 - 1, a);
-    fetch_phi(k0 - 1, fa[0], fb[0]);
     fetch(k0, b);
-    fetch_phi(k0, fa[1], fb[1]);
     fetch(k0 + 1, c);
-    fetch_phi(k0 + 1, fa[2], fb[2]);
-    commit(k0 - 1, a, fa[0], fb[0]);
-    commit(k0, b, fa[1], fb[1]);
-    commit(k0 + 1, c, fa[2], fb[2]);
+    commit(k0 - 1, a);
+    commit(k0, b);
+    commit(k0 + 1, c);
   }
   T pre[PR];
-  T pfa[PE], pfb[PE];
   T pq[PQS];
-  if (!AHEAD && k0 + 2 <= k1) {
-    fetch(k0 + 2, pre);
-    fetch_phi(k0 + 2, pfa, pfb);
-    pfetch(k0 + 2, pq);
-  }
+  if (!AHEAD && k0 + 2 <= k1) fetch(k0 + 2, pre);
+  if (k0 + 2 <= k1) pfetch(k0 + 2, pq);
   // PL: phi below this thread's point on the plane before the current one
   // (carried from plane to plane)
   T fprev = (T)0;
@@ -535,10 +433,9 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   for (int k = k0; k < k1; k++) {
     if constexpr (!AHEAD && NS >= 4) {
       // the slot of plane k-2, which no thread reads any more
-      if (k + 2 <= k1) commit(k + 2, pre, pfa, pfb);
+      if (k + 2 <= k1) commit(k + 2, pre);
       if (k + 3 <= k1) {
         fetch(k + 3, pre);
-        fetch_phi(k + 3, pfa, pfb);
         pfetch(k + 3, pq);
       }
     }
@@ -556,15 +453,6 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
           if (inside[c]) pm[c] = p[-doff[c]];
         if constexpr (!PCARRY)
           if (inside[2]) pm[2] = p[-doff[2]];
-        if constexpr (PROJ && !PL) {
-          const T* ph = phi + ol;
-          f0 = ph[0];
-#pragma unroll
-          for (int c = 0; c < 2; c++)
-            if (inside[c]) fm[c] = ph[-doff[c]];
-          if constexpr (!PCARRY)
-            if (inside[2]) fm[2] = ph[-doff[2]];
-        }
         if constexpr (KIND >= 2) {
 #pragma unroll
           for (int c = 0; c < 3; c++)
@@ -588,7 +476,9 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
     const bool more = k + 2 <= k1;
     if constexpr (AHEAD) {
       fetch(min(k + 2, k1), pre);
-      fetch_phi(min(k + 2, k1), pfa, pfb);
+      // PL: phi plane k+3, which the projection of plane k+3 reads one plane
+      // later (and past the chunk's last plane a repeat, not committed)
+      pfetch(min(k + 3, k1), pq);
     }
     // AHEAD, the full step computes every component on every thread (the
     // taps of a thread outside the field still lie in the LDS tile; only the
@@ -693,7 +583,11 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
     // thread has read it since the previous barrier)
     if constexpr (AHEAD) {
       if constexpr (NS == 3) __syncthreads();
-      if (more) commit(k + 2, pre, pfa, pfb);
+      if (more) commit(k + 2, pre);
+      // PL: phi plane k+3 into the slot of plane k-1 (the pressure carried
+      // from plane to plane leaves plane k-1 of phi unread here)
+      if constexpr (PL)
+        if (k + 3 <= k1) pcommit(k + 3, pq);
       __syncthreads();
     } else {
       // PL: phi plane k+3 into the slot of plane k-1 (read last by plane
@@ -703,11 +597,8 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       __syncthreads();
       if constexpr (NS == 3) {
         // plane k+2 replaces plane k-1; plane k+3 starts loading
-        if (more) commit(k + 2, pre, pfa, pfb);
-        if (k + 3 <= k1) {
-          fetch(k + 3, pre);
-          fetch_phi(k + 3, pfa, pfb);
-        }
+        if (more) commit(k + 2, pre);
+        if (k + 3 <= k1) fetch(k + 3, pre);
         __syncthreads();
       }
     }
