@@ -28,10 +28,10 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic words per pressure unknown per launch of each kernel class:
 # each logical array read once and written once per kernel (stencil
 # neighbour re-reads and ghost planes counted once; O(N^2) boundary work
-# excluded).  Stage launches alternate 10, 13, 10 words (mean 11); in the
-# FP64 form, whose stages 2 and 3 carry the previous stage's projection
-# (phi read, pressure written), 10, 15, 12 (mean 37/3), and one projection
-# launch per step remains.
+# excluded).  Stage launches alternate 10, 13, 10 words (mean 11) with the
+# projections unfused; the timestep fuses the projections of stages 1 and 2
+# into stages 2 and 3 (phi read, pressure written): 10, 15, 12 (mean 37/3),
+# and one projection launch per step remains.
 WORDS_PER_LAUNCH = {
     "stage": 11.0,
     "div_rhs": 4.0,
@@ -146,8 +146,8 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, 
         "unknowns": n * (n + 1) * n,
         "esize": 8 if precision == mif.FP64 else 4,
         # the library fuses the projections of stages 1 and 2 into the next
-        # stage kernel in the FP64 form
-        "fused_projection": precision == mif.FP64,
+        # stage kernel (both forms)
+        "fused_projection": True,
         "kernels": kernels,
         "velocity": final,
         "pressure": pres,
