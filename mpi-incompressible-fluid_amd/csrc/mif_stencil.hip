@@ -168,15 +168,18 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   constexpr int NS = PROJ && sizeof(T) == 8 ? kStageSlotsFused64 : (sizeof(T) == 4 ? kStageSlots32 : kStageSlots64);
   // p at plane k-1 carried in a register from plane to plane instead of
   // loaded (FP64: 256^3 stage kernel 348 -> 341 us, step 2.955 -> 2.928 ms;
-  // FP32: 181 -> 183 us on 64 x 8 tiles, 176 -> 179 us on 128 x 4, so FP32
-  // keeps the load)
-  constexpr bool PCARRY = sizeof(T) == 8;
-  // order of the memory requests per plane (see the plane loop)
-  constexpr bool AHEAD = sizeof(T) == 8;
+  // FP32: 181 -> 183 us on 64 x 8 tiles, 176 -> 179 us on 128 x 4, so the
+  // FP32 kernel of stage 1 keeps the load)
+  constexpr bool PCARRY = sizeof(T) == 8 || PROJ;
+  // order of the memory requests per plane (see the plane loop); the FP32
+  // kernels with the fused projection take the FP64 order and the carried
+  // pressure too (256^3 mixed step 1.437 -> 1.406 ms, fused kinds about 231
+  // -> 211 us)
+  constexpr bool AHEAD = sizeof(T) == 8 || PROJ;
   // fused projection: the phi planes are staged through an LDS ring, so that
   // phi is read once per plane instead of at every halo element and below it
   constexpr bool PL = PROJ;
-  static_assert(!PL || AHEAD || NS >= 4, "the FP32 order runs the phi ring with one barrier per plane");
+  static_assert(!PL || AHEAD, "the phi ring runs in the order that fetches plane k+2 during plane k");
   using TT = StageTile<T>;
   constexpr int STX = TT::STX, STY = TT::STY, SHX = TT::SHX, SHY = TT::SHY, STN = TT::STN;
   constexpr int PLANE_ELEMS = TT::PLANE_ELEMS, PQ = TT::PQ;
@@ -393,14 +396,10 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   T pre[PR];
   T pq[PQS];
   if (!AHEAD && k0 + 2 <= k1) fetch(k0 + 2, pre);
-  if (k0 + 2 <= k1) pfetch(k0 + 2, pq);
-  // PL: phi below this thread's point on the plane before the current one
-  // (carried from plane to plane)
-  T fprev = (T)0;
   if constexpr (PL) {
-    fprev = phl[(k0 - 1) % NSP][(ty + 2) * PHX + tx + 2];
-    // plane k0+2 takes the slot of plane k0-2 once every projection above
-    // is done
+    // phi plane k0+2 takes the slot of plane k0-2 once every projection
+    // above is done
+    if (k0 + 2 <= k1) pfetch(k0 + 2, pq);
     __syncthreads();
     if (k0 + 2 <= k1) pcommit(k0 + 2, pq);
   }
@@ -434,10 +433,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
     if constexpr (!AHEAD && NS >= 4) {
       // the slot of plane k-2, which no thread reads any more
       if (k + 2 <= k1) commit(k + 2, pre);
-      if (k + 3 <= k1) {
-        fetch(k + 3, pre);
-        pfetch(k + 3, pq);
-      }
+      if (k + 3 <= k1) fetch(k + 3, pre);
     }
     const long o = L.idx(i, j, k);
     const long ol = AHEAD ? oc + (long)k * L.plane : o;
@@ -493,13 +489,14 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       // p + (phi - mean) (k_project's expression), at the point and below it
       T pc = p0;
       if constexpr (PL) {
+        // phi at the point and below it along x and y (along z the carried
+        // pressure has it)
+        static_assert(!PL || PCARRY, "the fused kernels carry p + phi from plane to plane");
         const T* cur = &phl[k % NSP][0];
         const int ix = (ty + 2) * PHX + tx + 2;
         f0 = cur[ix];
         fm[0] = cur[ix - 1];
         fm[1] = cur[ix - PHX];
-        fm[2] = fprev;
-        fprev = f0;
       }
       if constexpr (PROJ) {
         pc = p0 + (f0 - mv);
@@ -590,10 +587,6 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
         if (k + 3 <= k1) pcommit(k + 3, pq);
       __syncthreads();
     } else {
-      // PL: phi plane k+3 into the slot of plane k-1 (read last by plane
-      // k-1's pointwise terms)
-      if constexpr (PL)
-        if (k + 3 <= k1) pcommit(k + 3, pq);
       __syncthreads();
       if constexpr (NS == 3) {
         // plane k+2 replaces plane k-1; plane k+3 starts loading
