@@ -167,15 +167,15 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // computed; one barrier per plane)
   constexpr int NS = PROJ && sizeof(T) == 8 ? kStageSlotsFused64 : (sizeof(T) == 4 ? kStageSlots32 : kStageSlots64);
   // p at plane k-1 carried in a register from plane to plane instead of
-  // loaded (FP64: 256^3 stage kernel 348 -> 341 us, step 2.955 -> 2.928 ms;
-  // FP32: 181 -> 183 us on 64 x 8 tiles, 176 -> 179 us on 128 x 4, so the
-  // FP32 kernel of stage 1 keeps the load)
-  constexpr bool PCARRY = sizeof(T) == 8 || PROJ;
-  // order of the memory requests per plane (see the plane loop); the FP32
-  // kernels with the fused projection take the FP64 order and the carried
-  // pressure too (256^3 mixed step 1.437 -> 1.406 ms, fused kinds about 231
-  // -> 211 us)
-  constexpr bool AHEAD = sizeof(T) == 8 || PROJ;
+  // loaded (FP64: 256^3 stage kernel 348 -> 341 us, step 2.955 -> 2.928 ms)
+  constexpr bool PCARRY = sizeof(T) == 8 || KIND <= 3;
+  // order of the memory requests per plane (see the plane loop).  The FP32
+  // full-step kernels took the FP64 order and the carried pressure with the
+  // field-major halo slots (below): fused kinds about 231 -> 211 us (256^3
+  // mixed step 1.437 -> 1.406 ms), stage 1 about 177 -> 166 us (1.394 ->
+  // 1.382 ms); with the element-major slots the same order had measured
+  // slower (stage 1 175 -> 181 us)
+  constexpr bool AHEAD = sizeof(T) == 8 || KIND <= 3;
   // fused projection: the phi planes are staged through an LDS ring, so that
   // phi is read once per plane instead of at every halo element and below it
   constexpr bool PL = PROJ;
@@ -212,7 +212,10 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // point q = tid + STN s of the SHY x SHX plane, the same point of u, v and
   // w, so that the three share its phi values and its index arithmetic
   constexpr int FQ = (SHY * SHX + STN - 1) / STN;
-  constexpr int PR = PL ? 3 * FQ : (VEC ? QV * VW + 1 : PQ);  // plane values a thread holds
+  // field-major slots: the fused kernels and the FP32 full-step kernels (the
+  // FP64 kernels of stage 1 keep their 16-byte loads: the same time)
+  constexpr bool FM = PL || (sizeof(T) == 4 && KIND <= 3);
+  constexpr int PR = FM ? 3 * FQ : (VEC ? QV * VW + 1 : PQ);  // plane values a thread holds
   typedef T Vec __attribute__((ext_vector_type(VW)));
   // AHEAD, halo loads go through global-address-space pointers: generic
   // (flat) loads also count as LDS operations, so every wait for the LDS
@@ -259,13 +262,13 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
     }
   }
   const T* phi = (const T*)s.phi;
-  // PL: per slot the source offset of its point (a point outside the
+  // FM: per slot the source offset of its point (a point outside the
   // arrays reads element 0: its value is masked), whether it is inside, its
-  // (i, j) and its index in the LDS phi plane
-  constexpr int FQS = PL ? FQ : 1;
+  // (i, j) and (PL) its index in the LDS phi plane
+  constexpr int FQS = FM ? FQ : 1;
   int xo[FQS], xgi[FQS], xgj[FQS], xix[FQS];
   bool xok[FQS];
-  if constexpr (PL) {
+  if constexpr (FM) {
 #pragma unroll
     for (int r = 0; r < FQ; r++) {
       const int q = tid + STN * r;
@@ -315,7 +318,7 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // instruction waits for the loads before the commit) or as they arrive.
   auto mask = [&](bool keep, T x) { return keep ? x : (T)0; };
   auto fetch = [&](int k, T (&v)[PR]) {
-    if constexpr (PL) {
+    if constexpr (FM) {
 #pragma unroll
       for (int r = 0; r < FQ; r++)
 #pragma unroll
@@ -339,7 +342,14 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   };
   auto commit = [&](int k, const T (&v)[PR]) {
     T* base = &pl[k % NS][0][0][0];
-    if constexpr (PL) {
+    if constexpr (FM && !PL) {
+#pragma unroll
+      for (int r = 0; r < FQ; r++) {
+        if (tid + STN * r >= SHY * SHX) continue;
+#pragma unroll
+        for (int f = 0; f < 3; f++) base[f * (SHY * SHX) + tid + STN * r] = mask(xok[r], v[f * FQ + r]);
+      }
+    } else if constexpr (PL) {
       // projected values X - pdts grad(phi) on each field's interior, phi at
       // the point and below it from the LDS planes k and k-1 (k >= 0 here)
       const T* cur = &phl[k % NSP][0];
