@@ -6,7 +6,10 @@ periodic x and z, no-slip walls in y, domain [0,2pi]x[-1,1]x[0,pi], Re = 1000,
 dt = 5e-4), Poiseuille profile plus a seeded divergence-free perturbation of
 amplitude 1e-2, all fields resident in HBM before the timed region.  One step
 is a full RK3 timestep with a pressure projection per stage.  The metric is
-cell-updates/s = pressure unknowns (256 x 257 x 256) x steps / seconds.
+cell-updates/s = pressure unknowns (256 x 257 x 256) x steps / seconds, over
+blocks of steps each run by one mif_timestep_n call (every step complete;
+each step's last projection rides in the next step's first stage kernel);
+the same blocks as single mif_timestep calls are reported beside it.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits with an explanation: this
 build runs the whole domain on one GPU.
@@ -56,14 +59,28 @@ BYTES_FP64_TRANSFORMS = {
 }
 
 
-FUSED_STAGE_WORDS = 37.0 / 3.0
+# Stage launches of one mif_timestep_n call of M steps, by (kind, fused
+# projection): the first step's stage 1 has no projection to apply, every
+# later step's stage 1 applies the previous step's last one; stages 2 and 3
+# always apply their predecessor's.  Words per launch of each, and the
+# arrays read and written (the streaming probe's mix).
+STAGE_KINDS = {(1, False): (10, (4, 6)), (1, True): (12, (5, 7)), (2, True): (15, (8, 7)), (3, True): (12, (8, 4))}
+
+
+def stage_launch_counts(m: int):
+    return {(1, False): 1, (1, True): m - 1, (2, True): m, (3, True): m}
+
+
+def stage_words(m: int) -> float:
+    n = stage_launch_counts(m)
+    return sum(STAGE_KINDS[k][0] * c for k, c in n.items()) / sum(n.values())
 
 
 def bytes_per_unknown(name: str, res) -> float:
     if res["esize"] == 4 and res.get("solve_transforms") == "fp64" and name in BYTES_FP64_TRANSFORMS:
         return BYTES_FP64_TRANSFORMS[name]
-    if name == "stage" and res.get("fused_projection"):
-        return FUSED_STAGE_WORDS * res["esize"]
+    if name == "stage" and res.get("timing_pass_steps"):
+        return stage_words(res["timing_pass_steps"]) * res["esize"]
     return WORDS_PER_LAUNCH[name] * res["esize"]
 
 
@@ -97,8 +114,20 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, 
         t += ctx.dt
     ctx.synchronize()
     # the timed steps run as REPEATS contiguous blocks, each bracketed by a
-    # device synchronisation; the step time is the median over the blocks
+    # device synchronisation and each one mif_timestep_n call (the steps of a
+    # block in one call: every step's last projection rides in the next
+    # step's first stage kernel); the step time is the median over the blocks
     blocks = []
+    for nb in block_sizes(steps):
+        ctx.synchronize()
+        b0 = time.perf_counter()
+        mif.timestep_n(vel, buf, buf2, ev, t, p, pb, nb)
+        t += nb * ctx.dt
+        ctx.synchronize()
+        blocks.append((time.perf_counter() - b0) / nb)
+    # the same blocks as single mif_timestep calls (the reference's call
+    # pattern: one timestep per call)
+    single = []
     for nb in block_sizes(steps):
         ctx.synchronize()
         b0 = time.perf_counter()
@@ -106,7 +135,7 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, 
             mif.timestep(vel, buf, buf2, ev, t, p, pb)
             t += ctx.dt
         ctx.synchronize()
-        blocks.append((time.perf_counter() - b0) / nb)
+        single.append((time.perf_counter() - b0) / nb)
     import numpy as np
 
     final = vel.download() if keep_fields else None
@@ -120,9 +149,9 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, 
     # per-kernel breakdown: a separate pass with HIP events around every
     # launch (kept out of the timed steps above, which it would slow down)
     ctx.set_timing(True)
-    for _ in range(max(2, min(steps, 5))):
-        mif.timestep(vel, buf, buf2, ev, t, p, pb)
-        t += ctx.dt
+    timing_steps = max(2, min(steps, 5))
+    mif.timestep_n(vel, buf, buf2, ev, t, p, pb, timing_steps)
+    t += timing_steps * ctx.dt
     ctx.synchronize()
     kernels = {}
     for cls, name in enumerate(mif.KERNEL_CLASSES):
@@ -143,10 +172,12 @@ def run_gpu(n: int, steps: int, warmup: int, precision: int, sweeps: int, init, 
         "seconds": sum(blocks[i] * nb for i, nb in enumerate(block_sizes(steps))),
         "step_s": float(np.median(blocks)),
         "block_step_s": blocks,
+        "single_call_step_s": float(np.median(single)),
+        "timing_pass_steps": timing_steps,
         "unknowns": n * (n + 1) * n,
         "esize": 8 if precision == mif.FP64 else 4,
-        # the library fuses the projections of stages 1 and 2 into the next
-        # stage kernel (both forms)
+        # the library fuses each stage's projection into the next stage
+        # kernel (both forms), across steps within one mif_timestep_n call
         "fused_projection": True,
         "kernels": kernels,
         "velocity": final,
@@ -417,10 +448,10 @@ def roofline_of(res):
     return roof, table
 
 
-# arrays read and written per launch of a kernel class (one entry per launch
-# kind): the stage kinds 1, 2, 3 (unfused, and with the projection fused into
-# kinds 2 and 3) and the projection
-STREAM_MIX = {"stage": [(4, 6), (7, 6), (7, 3)], "stage_fused": [(4, 6), (8, 7), (8, 4)], "project": [(5, 4)]}
+# arrays read and written per launch of a kernel class, with each launch
+# kind's share of the class's launches: the unfused stage kinds 1, 2, 3 and
+# the projection (the fused stage kinds: STAGE_KINDS)
+STREAM_MIX = {"stage": [(4, 6, 1), (7, 6, 1), (7, 3, 1)], "project": [(5, 4, 1)]}
 
 
 def stream_ceiling(cls: str, res, roof):
@@ -432,17 +463,19 @@ def stream_ceiling(cls: str, res, roof):
 
     if cls not in STREAM_MIX:
         return None
-    if cls == "stage" and res.get("fused_projection"):
-        cls = "stage_fused"
+    mixes = STREAM_MIX[cls]
+    if cls == "stage" and res.get("timing_pass_steps"):
+        n = stage_launch_counts(res["timing_pass_steps"])
+        mixes = [(*STAGE_KINDS[k][1], c) for k, c in n.items() if c > 0]
     nbytes = res["unknowns"] * res["esize"]
     ctx = mif.Context(channel_grid(8, 8, 8), precision=mif.FP64, device=0)
-    us = [ctx.stream_probe(r, w, nbytes, reps=10) for r, w in STREAM_MIX[cls]]
+    us = [ctx.stream_probe(r, w, nbytes, reps=10) for r, w, _ in mixes]
     ctx.close()
-    t = sum(us) / len(us)
+    t = sum(u * c for u, (_, _, c) in zip(us, mixes)) / sum(c for _, _, c in mixes)
     gbs = roof["alg_bytes_per_launch"] / (t * 1e-6) / 1e9
     return {"GB/s": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
             "achieved_over_ceiling": round(roof["achieved"] / gbs, 4),
-            "mixes": ", ".join(f"{r}r+{w}w" for r, w in STREAM_MIX[cls]), "array_bytes": nbytes,
+            "mixes": ", ".join(f"{r}r+{w}w x{c}" for r, w, c in mixes), "array_bytes": nbytes,
             "probe_us": [round(x, 1) for x in us]}
 
 
@@ -457,10 +490,12 @@ CLASS_KERNELS = {
 }
 
 
-def measured_traffic(cls: str, esize: int):
+def measured_traffic(cls: str, esize: int, timing_steps: int = 0):
     """HBM bytes per launch of a kernel class at the 256^3 workload (FP64 or
     FP32 kernels by esize), from the committed PMC summary (FETCH_SIZE and
-    WRITE_SIZE scaled by the plain copy calibration)."""
+    WRITE_SIZE scaled by the plain copy calibration).  The stage class is
+    weighted by its kinds' shares of the timing pass (timing_steps steps in
+    one mif_timestep_n call)."""
     import re
 
     if cls not in CLASS_KERNELS or not os.path.exists(TRAFFIC_PROFILE):
@@ -469,7 +504,18 @@ def measured_traffic(cls: str, esize: int):
     if not last:
         return None
     data = json.loads(last[-1])["kernels"]
-    pat = CLASS_KERNELS[cls].replace("{T}", "double" if esize == 8 else "float")
+    tname = "double" if esize == 8 else "float"
+    if cls == "stage" and timing_steps:
+        per_kind = {}
+        for k, v in data.items():
+            m = re.search(rf"k_stage<{tname}, ([123]), \w+, \w+, \w+, (true|false)>", k)
+            if m:
+                per_kind[(int(m.group(1)), m.group(2) == "true")] = v["total_bytes"]
+        n = stage_launch_counts(timing_steps)
+        if any(c > 0 and kind not in per_kind for kind, c in n.items()):
+            return None
+        return round(sum(per_kind[kind] * c for kind, c in n.items() if c > 0) / sum(n.values()))
+    pat = CLASS_KERNELS[cls].replace("{T}", tname)
     vals = [v["total_bytes"] for k, v in data.items() if re.search(pat, k)]
     return round(sum(vals) / len(vals)) if vals else None
 
@@ -527,7 +573,7 @@ def copy_calibration(mib: int):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="intervals per axis")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample (0 = skip)")
@@ -581,6 +627,7 @@ def main():
         forms[name] = {"cell_updates_per_s": rate, "ms_per_step": r["step_s"] * 1e3,
                        "ms_per_step_blocks": [round(b * 1e3, 4) for b in r["block_step_s"]],
                        "ms_per_step_mean": r["seconds"] / args.steps * 1e3,
+                       "ms_per_step_single_calls": r["single_call_step_s"] * 1e3,
                        "finite": bool(all(np.isfinite(c).all() for c in r["velocity"])),
                        "kernel_avg_us": {k: round(v["avg_ms"] * 1e3, 2) for k, v in r["kernels"].items()}}
         if name != "fp64":
@@ -610,7 +657,7 @@ def main():
     value = forms[chosen]["cell_updates_per_s"]
     roof, table = roofline_of(res)
     if roof is not None and args.n == 256:
-        roof["traffic"] = measured_traffic(roof["kernel"], res["esize"])
+        roof["traffic"] = measured_traffic(roof["kernel"], res["esize"], res.get("timing_pass_steps", 0))
     if roof is not None:
         print("[bench] streaming ceiling of the dominant kernel's mix", file=sys.stderr, flush=True)
         roof["stream_ceiling"] = stream_ceiling(roof["kernel"], res, roof)
@@ -650,7 +697,9 @@ def main():
         "config": {"workload": f"channel_timestep_{args.n}^3", "grid_nodes": [args.n + 1] * 3,
                    "pressure_unknowns": res["unknowns"], "periodic": "x,z", "walls": "y", "Re": 1000.0,
                    "dt": 5e-4, "form": chosen, "timing": f"median over {len(block_sizes(args.steps))} "
-                                                         f"synchronised blocks of the {args.steps} timed steps"},
+                                                         f"synchronised blocks of the {args.steps} timed steps, "
+                                                         f"each block one mif_timestep_n call",
+                   "ms_per_step_single_calls": forms[chosen]["ms_per_step_single_calls"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "forms": forms,

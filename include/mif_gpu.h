@@ -172,6 +172,15 @@ int mif_velocity_apply_bc_tabulated(mif_field *const vel[3], const double *faces
 int mif_timestep(mif_field *const vel[3], mif_field *const buf[3], mif_field *const buf2[3],
                  const mif_function *exact_velocity, double t_n, mif_field *pressure,
                  mif_field *pressure_buffer);
+/* nsteps successive mif_timestep calls at t_n, t_n + dt, ...: the result is
+ * that of the loop (to rounding).  No reference counterpart (the reference's
+ * driver loops over timestep, src/main.cpp); a step's last projection is
+ * applied by the next step's first stage kernel instead of a pass of its
+ * own, except after the last step.  With stage records on
+ * (mif_ctx_set_stage_records) it is that loop. */
+int mif_timestep_n(mif_field *const vel[3], mif_field *const buf[3], mif_field *const buf2[3],
+                   const mif_function *exact_velocity, double t_n, mif_field *pressure,
+                   mif_field *pressure_buffer, int nsteps);
 /* timestep_nhn (include/Timestep.h:23-27, src/Timestep.cpp:143-149): the
  * same with non-homogeneous Neumann data from the exact pressure gradient
  * (all-wall domains only). */

@@ -650,13 +650,18 @@ int apply_bc(mif_ctx* c, mif_field* const v[3], const FnDesc& f, double t) {
   return MIF_OK;
 }
 
+// pout: where p + phi goes (default: p itself)
 template <typename T>
-int project(mif_ctx* c, mif_field* const v[3], mif_field* p, mif_field* phi, double dts) {
+int project(mif_ctx* c, mif_field* const v[3], void* p, mif_field* phi, double dts, void* pout = nullptr) {
   KTimer tm(c, MIF_KC_PROJECT);
-  launch_project<T>(c->g, c->L, fptr<T>(v[0]), fptr<T>(v[1]), fptr<T>(v[2]), fptr<T>(p), fptr<T>(phi), dts,
-                    c->stream, c->phi_mean);
+  launch_project<T>(c->g, c->L, fptr<T>(v[0]), fptr<T>(v[1]), fptr<T>(v[2]), (T*)p, fptr<T>(phi), dts, c->stream,
+                    c->phi_mean, (T*)pout);
   c->phi_mean = nullptr;
   return MIF_OK;
+}
+template <typename T>
+int project(mif_ctx* c, mif_field* const v[3], mif_field* p, mif_field* phi, double dts) {
+  return project<T>(c, v, p->data, phi, dts);
 }
 
 // Stage times of FULL_TIMESTEP (src/Timestep.cpp:99-104): the boundary data
@@ -679,12 +684,25 @@ StageTimes stage_times(double dt, double t_n) {
   return st;
 }
 
+// State carried from step to step by a multi-step call (fused form): the
+// pressure buffer that holds p (0: the field, 1: the context's second
+// pressure buffer) and, when pending_dts > 0, the previous step's stage-3
+// projection (phi in pb, its mean in c->phi_mean), which the next step's
+// stage-1 kernel applies.
+struct StepCarry {
+  int cur = 0;
+  double pending_dts = 0;
+};
+
 // FULL_TIMESTEP (src/Timestep.cpp:97-141).  ev[s - 1] is the boundary data
 // of stage s (a descriptor evaluated at the stage time, or a host table);
-// gd, when not null, the Neumann data of the three stages' solves.
+// gd, when not null, the Neumann data of the three stages' solves.  carry,
+// last: a multi-step call (mif_timestep_n); every step but the last leaves
+// its stage-3 projection to the next step's stage-1 kernel.
 template <typename T>
 int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
-                  const FnDesc ev[3], const FnDesc* gd, double t_n, mif_field* p, mif_field* pb) {
+                  const FnDesc ev[3], const FnDesc* gd, double t_n, mif_field* p, mif_field* pb,
+                  StepCarry* carry = nullptr, bool last = true) {
   const double dt = c->g.dt;
   const StageTimes st = stage_times(dt, t_n);
   const int nhn = gd != nullptr;
@@ -693,12 +711,17 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   // Fused form: the projections of stages 1 and 2 (velocity correction and
   // pressure increment) are applied by the stage kernels of stages 2 and 3
   // as they read Y2, Y3 and p (one pass over the velocity and pressure fewer
-  // per fused projection).  The pressure increments alternate between p and
-  // the context's second pressure buffer and end in p after stage 3.
+  // per fused projection).  Each fused stage reads p from one of two buffers
+  // (p and the context's second pressure buffer) and writes the incremented
+  // pressure into the other; the last projection of the call writes p.
   const bool fuse = !c->stage_records;
   if (fuse && !c->pswap) HIPCHK(c, alloc_field_storage(c, &c->pswap));
-  void* const pd = p->data;
-  void* const ps = c->pswap;
+  StepCarry single;
+  StepCarry& cs = carry ? *carry : single;
+  void* const pbuf[2] = {p->data, c->pswap};
+  // the pressure buffer a fused stage reads, then (after it) the one it wrote
+  auto pin = [&]() { return pbuf[cs.cur]; };
+  auto pnext = [&]() { return pbuf[cs.cur ^ 1]; };
   // Pressure solve of stage k; with MIF_DEFECT_LAST_STAGE the mixed form's
   // defect sweeps run in the solve of stage 3 only.
   auto solve = [&](int k, mif_field* const* u) {
@@ -708,10 +731,18 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
     c->sweeps = sweeps;
     return r;
   };
-  // Stage 1 (src/Timestep.cpp:107-117)
+  // Stage 1 (src/Timestep.cpp:107-117), with the previous step's last
+  // projection when a multi-step call left it pending
   {
     const double a1 = 64.0 / 120.0 * dt;
-    if ((rc = stage<T>(c, 1, vel, nullptr, buf, buf2, pd, a1, 0.0, a1))) return rc;
+    if (fuse && cs.pending_dts > 0) {
+      rc = stage<T>(c, 1, vel, nullptr, buf, buf2, pin(), a1, 0.0, a1, pb->data, cs.pending_dts, pnext());
+      cs.cur ^= 1;
+      cs.pending_dts = 0;
+    } else {
+      rc = stage<T>(c, 1, vel, nullptr, buf, buf2, pin(), a1, 0.0, a1);
+    }
+    if (rc) return rc;
   }
   apply_bc<T>(c, buf, ev[0], st.time[1]);
   if ((rc = solve(1, buf))) return rc;
@@ -719,10 +750,12 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   // Stage 2 (src/Timestep.cpp:119-129)
   {
     const double a1 = -34.0 / 120.0 * dt, a2 = 50.0 / 120.0 * dt;
-    if (fuse)
-      rc = stage<T>(c, 2, buf, buf2, buf2, vel, pd, a1, a2, a1 + a2, pb->data, st.dt_s[1], ps);
-    else
-      rc = stage<T>(c, 2, buf, buf2, buf2, vel, pd, a1, a2, a1 + a2);
+    if (fuse) {
+      rc = stage<T>(c, 2, buf, buf2, buf2, vel, pin(), a1, a2, a1 + a2, pb->data, st.dt_s[1], pnext());
+      cs.cur ^= 1;
+    } else {
+      rc = stage<T>(c, 2, buf, buf2, buf2, vel, pin(), a1, a2, a1 + a2);
+    }
     if (rc) return rc;
   }
   apply_bc<T>(c, buf2, ev[1], st.time[2]);
@@ -731,15 +764,23 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
   // Stage 3 (src/Timestep.cpp:131-140)
   {
     const double a2 = -50.0 / 120.0 * dt, a3 = 90.0 / 120.0 * dt;
-    if (fuse)
-      rc = stage<T>(c, 3, buf2, vel, vel, nullptr, ps, 0.0, a3, a2 + a3, pb->data, st.dt_s[2], pd);
-    else
-      rc = stage<T>(c, 3, buf2, vel, vel, nullptr, pd, 0.0, a3, a2 + a3);
+    if (fuse) {
+      rc = stage<T>(c, 3, buf2, vel, vel, nullptr, pin(), 0.0, a3, a2 + a3, pb->data, st.dt_s[2], pnext());
+      cs.cur ^= 1;
+    } else {
+      rc = stage<T>(c, 3, buf2, vel, vel, nullptr, pin(), 0.0, a3, a2 + a3);
+    }
     if (rc) return rc;
   }
   apply_bc<T>(c, vel, ev[2], st.time[3]);
   if ((rc = solve(3, vel))) return rc;
-  project<T>(c, vel, p, pb, st.dt_s[3]);
+  if (fuse && !last) {
+    cs.pending_dts = st.dt_s[3];
+  } else {
+    // the step's last projection, its pressure written into p
+    project<T>(c, vel, pin(), pb, st.dt_s[3], p->data);
+    cs.cur = 0;
+  }
   LAUNCHCHK(c);
   return MIF_OK;
 }
@@ -1539,6 +1580,26 @@ int mif_timestep(mif_field* const vel[3], mif_field* const buf[3], mif_field* co
   const FnDesc d = make_fn(exact_velocity);
   const FnDesc ev[3] = {d, d, d};
   return run_timestep(c, vel, buf, buf2, ev, nullptr, t_n, p, pb);
+}
+
+int mif_timestep_n(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],
+                   const mif_function* exact_velocity, double t_n, mif_field* p, mif_field* pb, int nsteps) {
+  mif_ctx* c = nullptr;
+  int rc = check_timestep(vel, buf, buf2, p, pb, false, &c);
+  if (rc) return rc;
+  if (!exact_velocity) return fail(c, MIF_ERR_INVALID_ARGUMENT, "null exact velocity");
+  if (nsteps < 0) return fail(c, MIF_ERR_INVALID_ARGUMENT, "negative step count %d", nsteps);
+  const FnDesc d = make_fn(exact_velocity);
+  const FnDesc ev[3] = {d, d, d};
+  StepCarry cs;
+  for (int s = 0; s < nsteps; s++) {
+    const double t = t_n + s * c->g.dt;
+    const bool last = s == nsteps - 1;
+    rc = is_fp64(c) ? full_timestep<double>(c, vel, buf, buf2, ev, nullptr, t, p, pb, &cs, last)
+                    : full_timestep<float>(c, vel, buf, buf2, ev, nullptr, t, p, pb, &cs, last);
+    if (rc) return rc;
+  }
+  return MIF_OK;
 }
 
 int mif_timestep_nhn(mif_field* const vel[3], mif_field* const buf[3], mif_field* const buf2[3],

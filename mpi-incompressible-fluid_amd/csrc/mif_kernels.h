@@ -125,7 +125,7 @@ void launch_unpack(const DGrid& g, const Layout& L, const S* B, long bp0, long b
 template <typename T>
 // p += phi (minus mean->s1 / mean->s2 when mean is not null), velocity -= dts grad phi
 void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st,
-                    const Red3* mean = nullptr);
+                    const Red3* mean = nullptr, T* pout = nullptr);
 template <typename T>
 void launch_pgrad(const DGrid& g, const Layout& L, const T* p, T* u, T* v, T* w, hipStream_t st);
 template <typename T>

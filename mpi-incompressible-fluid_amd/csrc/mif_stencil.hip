@@ -161,7 +161,7 @@ template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = fal
 __global__ void __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(KIND <= 3 && (sizeof(T) == 8 || PROJ) ? (sizeof(T) == 8 && PROJ ? 4 : 6) : 1)))
 k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int kmax, int ntx, int nty, int ntiles) {
-  static_assert(!PROJ || KIND == 2 || KIND == 3, "a fused projection belongs to stages 2 and 3");
+  static_assert(!PROJ || KIND <= 3, "a fused projection belongs to the full-step kinds");
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 and
   // more (plane k+2 is committed into a free slot while plane k is
   // computed; one barrier per plane)
@@ -888,9 +888,12 @@ __global__ void __launch_bounds__(256) k_unpack(DGrid g, Layout L, const S* __re
 // while plane k is updated.  Loads read always-valid addresses and the
 // stores carry the point tests, so a plane's loads are issued together.
 template <typename T, bool NT>
+// The incremented pressure goes to pout: p itself, or another buffer (every
+// pressure point is written).
 __global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restrict__ u, T* __restrict__ v,
-                                                 T* __restrict__ w, T* __restrict__ p, const T* __restrict__ phi,
-                                                 double dts, int nqt, int kchunk, const Red3* __restrict__ mean) {
+                                                 T* __restrict__ w, const T* p, const T* __restrict__ phi,
+                                                 double dts, int nqt, int kchunk, const Red3* __restrict__ mean,
+                                                 T* pout) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int qt = tile % nqt, kt = tile / nqt;
   // slices start at the allocation's aligned origin, L.lead elements
@@ -924,7 +927,7 @@ __global__ void __launch_bounds__(256) k_project(DGrid g, Layout L, T* __restric
     Pt nxt = cur;
     if (k + 1 < k1) fetch(k + 1, nxt);
     const long o = (long)q + L.plane * k;
-    if (i < g.N[0] && j < g.N[1] && k < g.N[2]) store_out<NT>(&p[o], cur.pp + (cur.f - pm));
+    if (i < g.N[0] && j < g.N[1] && k < g.N[2]) store_out<NT>(&pout[o], cur.pp + (cur.f - pm));
     const T d[3] = {cur.f - cur.fx, cur.f - cur.fy, cur.f - fz};
 #pragma unroll
     for (int c = 0; c < 3; c++)
@@ -1130,8 +1133,9 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   };
   using yes = std::true_type;
   using no = std::false_type;
-  // kinds 2 and 3 take the previous stage's projection when s.phi is set
-  constexpr bool kFuse = KIND == 2 || KIND == 3;
+  // the full-step kinds take the previous stage's projection when s.phi is
+  // set (kind 1: the last projection of the previous step, multi-step calls)
+  constexpr bool kFuse = KIND <= 3;
   // the 16-byte loads need element 1 of every row on a 16-byte boundary:
   // the storage shift puts it on a 128-byte one, and rows are 64-byte
   // multiples.  Measured against 4- and 8-byte loads (step times): 256^3
@@ -1331,7 +1335,8 @@ void launch_unpack(const DGrid& g, const Layout& L, const S* B, long bp0, long b
 
 template <typename T>
 void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, const T* phi, double dts, hipStream_t st,
-                    const Red3* mean) {
+                    const Red3* mean, T* pout) {
+  if (!pout) pout = p;
   // points 0..N per axis (the union of the extents; u's x+ ghost included,
   // the interior tests filter), as 256-position slices of the padded plane
   // times chunks of 16 planes
@@ -1339,9 +1344,9 @@ void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, con
   const int nqt = (int)((L.plane + L.lead + 255) / 256);
   const int nkt = (g.N[2] + 1 + kchunk - 1) / kchunk;
   if (L.nt)
-    k_project<T, true><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk, mean);
+    k_project<T, true><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk, mean, pout);
   else
-    k_project<T, false><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk, mean);
+    k_project<T, false><<<nqt * nkt, 256, 0, st>>>(g, L, u, v, w, p, phi, dts, nqt, kchunk, mean, pout);
 }
 
 template <typename T>
@@ -1419,7 +1424,7 @@ void launch_gather(const Layout& L, const T* F, const int* ijk, long n, double* 
   template void launch_periodic<T>(const DGrid&, const Layout&, T*, int, hipStream_t);                              \
   template void launch_set<T>(const DGrid&, const Layout&, T*, int, int, const FnDesc&, double, int, hipStream_t);  \
   template void launch_project<T>(const DGrid&, const Layout&, T*, T*, T*, T*, const T*, double, hipStream_t,       \
-                                  const Red3*);                                                                     \
+                                  const Red3*, T*);                                                                 \
   template void launch_pgrad<T>(const DGrid&, const Layout&, const T*, T*, T*, T*, hipStream_t);                    \
   template void launch_from_dense<T>(const Layout&, const int*, const double*, T*, hipStream_t);                    \
   template void launch_to_dense<T>(const Layout&, const int*, const T*, double*, hipStream_t);                      \

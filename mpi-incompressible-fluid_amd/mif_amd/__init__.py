@@ -12,6 +12,6 @@ from .core import (Context, Field, GridSpec, MifError, VelocityField, adjoint_fo
                    error_norms_pressure,
                    error_norms_velocity, function, poisson_solve, pressure_gradient,
                    solve_pressure_equation_homogeneous_periodic, solve_pressure_equation_non_homogeneous_neumann,
-                   timestep, timestep_adjoint, timestep_nhn, timestep_velocity)
+                   timestep, timestep_adjoint, timestep_n, timestep_nhn, timestep_velocity)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

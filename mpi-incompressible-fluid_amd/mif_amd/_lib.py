@@ -100,6 +100,7 @@ SIGNATURES = {
     "mif_field_apply_periodic_bc": [_VP],
     "mif_velocity_apply_bc": [_V3, _FP, _D],
     "mif_timestep": [_V3, _V3, _V3, _FP, _D, _VP, _VP],
+    "mif_timestep_n": [_V3, _V3, _V3, _FP, _D, _VP, _VP, _I],
     "mif_timestep_nhn": [_V3, _V3, _V3, _FP, _FP, _D, _VP, _VP],
     "mif_timestep_velocity": [_V3, _V3, _V3, _FP, _FP, _D],
     "mif_solve_pressure_homogeneous_periodic": [_VP, _V3, _D],

@@ -416,6 +416,19 @@ def timestep(velocity: VelocityField, velocity_buffer: VelocityField, velocity_b
                                    ctypes.byref(exact_velocity), t_n, pressure.ptr, pressure_buffer.ptr))
 
 
+def timestep_n(velocity: VelocityField, velocity_buffer: VelocityField, velocity_buffer_2: VelocityField,
+               exact_velocity, t_n: float, pressure: Field, pressure_buffer: Field, nsteps: int):
+    """nsteps timestep() calls at t_n, t_n + dt, ... in one call
+    (mif_timestep_n): each step's last projection rides in the next step's
+    first stage kernel.  exact_velocity is a descriptor (function())."""
+    ctx = velocity.ctx
+    if _bd.is_callables(exact_velocity):
+        raise TypeError("timestep_n takes a function() descriptor; call timestep() per step for callables")
+    ctx.check(ctx.lib.mif_timestep_n(velocity.c_array(), velocity_buffer.c_array(), velocity_buffer_2.c_array(),
+                                     ctypes.byref(exact_velocity), t_n, pressure.ptr, pressure_buffer.ptr,
+                                     int(nsteps)))
+
+
 def timestep_nhn(velocity, velocity_buffer, velocity_buffer_2, exact_velocity, exact_pressure_gradient, t_n,
                  pressure, pressure_buffer):
     """timestep_nhn (include/Timestep.h:23-27).  Descriptors, or (both) three

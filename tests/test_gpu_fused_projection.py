@@ -84,3 +84,70 @@ def test_fused_matches_unfused_mixed(sweeps):
         assert rel_err(fv[c], uv[c]) < 1e-6, c
         assert np.all(np.isfinite(fv[c]))
     assert rel_err(fp, up) < 1e-4
+
+
+def _run_n(gs, init, nsteps, fn_kind, precision=None, records=False, multi=True):
+    kw = {} if precision is None else {"precision": precision}
+    ctx = mif.Context(gs, **kw)
+    ctx.set_stage_records(records)
+    vel, buf, buf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    p, pb = ctx.pressure(), ctx.pressure()
+    vel.upload(init[0])
+    p.upload(init[1])
+    fn = mif.function(fn_kind, Re=1e3)
+    if multi:
+        mif.timestep_n(vel, buf, buf2, fn, 0.0, p, pb, nsteps)
+    else:
+        for s in range(nsteps):
+            mif.timestep(vel, buf, buf2, fn, s * ctx.dt, p, pb)
+    out = [c for c in vel.download()], p.download()
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("nsteps", [1, 2, 3, 4])
+@pytest.mark.parametrize("ci", [0, 2], ids=["channel", "walls_es"])
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_timestep_n_matches_loop(form, ci, nsteps):
+    """mif_timestep_n (each step's last projection in the next step's first
+    stage kernel; the pressure ends in p for either parity of the buffer
+    swaps) against the loop of single steps."""
+    case = CASES[ci]
+    print(f"[case] timestep_n {form} {case[0]} nsteps={nsteps}", flush=True)
+    gs, o, init, _, fn = _initial(case)
+    prec = None if form == "fp64" else mif.MIXED
+    (nv, np_) = _run_n(gs, init, nsteps, fn, prec)
+    (lv, lp) = _run_n(gs, init, nsteps, fn, prec, multi=False)
+    tv, tp = (1e-13, 1e-12) if form == "fp64" else (1e-6, 1e-4)
+    for c in range(3):
+        assert rel_err(nv[c], lv[c]) < tv, c
+    assert rel_err(np_, lp) < tp
+
+
+def test_timestep_n_with_stage_records_is_the_loop():
+    case = CASES[0]
+    print(f"[case] timestep_n stage records {case[0]}", flush=True)
+    gs, o, init, _, fn = _initial(case)
+    (nv, np_) = _run_n(gs, init, 3, fn, records=True)
+    (lv, lp) = _run_n(gs, init, 3, fn, records=True, multi=False)
+    for c in range(3):
+        assert np.array_equal(nv[c], lv[c]), c
+    assert np.array_equal(np_, lp)
+
+
+def test_timestep_n_zero_and_negative():
+    case = CASES[0]
+    print("[case] timestep_n 0 and -1 steps", flush=True)
+    gs, o, init, _, fn = _initial(case)
+    ctx = mif.Context(gs)
+    vel, buf, buf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
+    p, pb = ctx.pressure(), ctx.pressure()
+    vel.upload(init[0])
+    p.upload(init[1])
+    f = mif.function(fn)
+    mif.timestep_n(vel, buf, buf2, f, 0.0, p, pb, 0)
+    for c, got in enumerate(vel.download()):
+        assert np.array_equal(got, init[0][c]), c
+    with pytest.raises(mif.MifError):
+        mif.timestep_n(vel, buf, buf2, f, 0.0, p, pb, -1)
+    ctx.close()
