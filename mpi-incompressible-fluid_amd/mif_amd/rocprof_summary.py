@@ -27,6 +27,8 @@ ALG_WORDS = [
     # template argument): phi read and the incremented pressure written too
     (r"k_stage<[^,]+, 2, \w+, \w+, \w+, true>", 15.0, "stage 2 + projection 1: Y2*(3),R1(3),p,phi -> Y3(3), S(3), p"),
     (r"k_stage<[^,]+, 3, \w+, \w+, \w+, true>", 12.0, "stage 3 + projection 2: Y3*(3),S(3),p,phi -> u(3), p"),
+    (r"k_stage<[^,]+, 1, \w+, \w+, \w+, true>", 12.0,
+     "stage 1 + the previous step's last projection: u*(3),p,phi -> Y2(3), R1(3), p"),
     (r"k_stage<[^,]+, 1(, \w+)*>", 10.0, "stage 1: u,v,w,p -> Y2(3), R1(3)"),
     (r"k_stage<[^,]+, 2(, \w+)*>", 13.0, "stage 2: Y2(3),R1(3),p -> Y3(3), S(3)"),
     (r"k_stage<[^,]+, 3(, \w+)*>", 10.0, "stage 3: Y3(3),S(3),p -> u(3)"),
