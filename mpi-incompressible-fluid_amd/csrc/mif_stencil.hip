@@ -126,7 +126,7 @@ struct StageTile {
 };
 // planes per workgroup.  FP32 on 128 x 4 tiles at 256^3: 8 planes 180.4 us,
 // 16 176.5 us, 24 178.0 us, 32 192.5 us.  FP64 on 64 x 8 tiles, with the
-// plane order of k_stage's AHEAD form: 4 planes 329 us, 6 328, 8 320, 10 320,
+// plane order of k_stage: 4 planes 329 us, 6 328, 8 320, 10 320,
 // 12 323, 16 336, 24 357, 32 342 (four ring slots instead of three: 8 planes
 // 318 us, 16 330 us, so three slots stay).  Fields beyond the Infinity Cache
 // (no non-temporal path, NT false) keep 16 planes in the full step: 512^3
@@ -169,17 +169,9 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // p at plane k-1 carried in a register from plane to plane instead of
   // loaded (FP64: 256^3 stage kernel 348 -> 341 us, step 2.955 -> 2.928 ms)
   constexpr bool PCARRY = sizeof(T) == 8 || KIND <= 3;
-  // order of the memory requests per plane (see the plane loop).  The FP32
-  // full-step kernels took the FP64 order and the carried pressure with the
-  // field-major halo slots (below): fused kinds about 231 -> 211 us (256^3
-  // mixed step 1.437 -> 1.406 ms), stage 1 about 177 -> 166 us (1.394 ->
-  // 1.382 ms); with the element-major slots the same order had measured
-  // slower (stage 1 175 -> 181 us)
-  constexpr bool AHEAD = sizeof(T) == 8 || KIND <= 3;
   // fused projection: the phi planes are staged through an LDS ring, so that
   // phi is read once per plane instead of at every halo element and below it
   constexpr bool PL = PROJ;
-  static_assert(!PL || AHEAD, "the phi ring runs in the order that fetches plane k+2 during plane k");
   using TT = StageTile<T>;
   constexpr int STX = TT::STX, STY = TT::STY, SHX = TT::SHX, SHY = TT::SHY, STN = TT::STN;
   constexpr int PLANE_ELEMS = TT::PLANE_ELEMS, PQ = TT::PQ;
@@ -212,20 +204,21 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   // point q = tid + STN s of the SHY x SHX plane, the same point of u, v and
   // w, so that the three share its phi values and its index arithmetic
   constexpr int FQ = (SHY * SHX + STN - 1) / STN;
-  // field-major slots: the fused kernels and the FP32 full-step kernels (the
-  // FP64 kernels of stage 1 keep their 16-byte loads: the same time)
-  constexpr bool FM = PL || (sizeof(T) == 4 && KIND <= 3);
+  // field-major slots: the fused kernels and every FP32 kernel (the FP64
+  // kernels without a projection keep their 16-byte loads: the FP64
+  // velocity-only step 1.49 -> 1.54 ms with field-major slots).  With them
+  // the FP32 kernels took the request order of the FP64 ones (see the plane
+  // loop), which with their earlier element-major slots had measured slower
+  // (stage 1 175 -> 181 us): 256^3 mixed fused kinds about 231 -> 211 us,
+  // stage 1 about 177 -> 166 us, velocity-only step 1.007 -> 0.931 ms.
+  constexpr bool FM = PL || sizeof(T) == 4;
   constexpr int PR = FM ? 3 * FQ : (VEC ? QV * VW + 1 : PQ);  // plane values a thread holds
   typedef T Vec __attribute__((ext_vector_type(VW)));
-  // AHEAD, halo loads go through global-address-space pointers: generic
-  // (flat) loads also count as LDS operations, so every wait for the LDS
-  // ring would wait for the plane in flight as well.  (The FP32 kernel, which
-  // waits for its halo plane before the stencil work anyway, measured 1 %
-  // faster with the generic loads.)
-  typedef const __attribute__((address_space(1))) T GT1;
-  typedef const __attribute__((address_space(1))) Vec GV1;
-  using GT = std::conditional_t<AHEAD, GT1, const T>;
-  using GV = std::conditional_t<AHEAD, GV1, const Vec>;
+  // halo loads go through global-address-space pointers: generic (flat)
+  // loads also count as LDS operations, so every wait for the LDS ring would
+  // wait for the plane in flight as well
+  typedef const __attribute__((address_space(1))) T GT;
+  typedef const __attribute__((address_space(1))) Vec GV;
   GT* src[VEC ? QV + 1 : PQ];
   int dst[VEC ? QV + 1 : PQ];
   bool ok[VEC ? QV + 1 : PQ];
@@ -314,8 +307,8 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   };
   // every plane read lies in 0 .. kmax + 1 <= SZ - 1, so the load address is
   // valid for every lane and the loads need no branch.  Lanes outside the
-  // field are zeroed when the values are committed to LDS (AHEAD: so that no
-  // instruction waits for the loads before the commit) or as they arrive.
+  // field are zeroed when the values are committed to LDS, so that no
+  // instruction waits for the loads before the commit.
   auto mask = [&](bool keep, T x) { return keep ? x : (T)0; };
   auto fetch = [&](int k, T (&v)[PR]) {
     if constexpr (FM) {
@@ -328,15 +321,15 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
       for (int r = 0; r < QV; r++) {
         const Vec x = *(GV*)(src[r] + (long)k * L.plane);
 #pragma unroll
-        for (int e = 0; e < VW; e++) v[r * VW + e] = AHEAD ? x[e] : mask(ok[r], x[e]);
+        for (int e = 0; e < VW; e++) v[r * VW + e] = x[e];
       }
       const T h = src[QV][(long)k * L.plane];
-      v[QV * VW] = AHEAD ? h : mask(ok[QV], h);
+      v[QV * VW] = h;
     } else {
 #pragma unroll
       for (int r = 0; r < PQ; r++) {
         const T x = src[r][(long)k * L.plane];
-        v[r] = AHEAD ? x : mask(ok[r], x);
+        v[r] = x;
       }
     }
   };
@@ -376,12 +369,12 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
         if (dst[r] >= 0)
 #pragma unroll
           for (int e = 0; e < VW; e++)
-            base[dst[r] + e] = AHEAD ? mask(ok[r], v[r * VW + e]) : v[r * VW + e];
-      if (dst[QV] >= 0) base[dst[QV]] = AHEAD ? mask(ok[QV], v[QV * VW]) : v[QV * VW];
+            base[dst[r] + e] = mask(ok[r], v[r * VW + e]);
+      if (dst[QV] >= 0) base[dst[QV]] = mask(ok[QV], v[QV * VW]);
     } else {
 #pragma unroll
       for (int r = 0; r < PQ; r++)
-        if (dst[r] >= 0) base[dst[r]] = AHEAD ? mask(ok[r], v[r]) : v[r];
+        if (dst[r] >= 0) base[dst[r]] = mask(ok[r], v[r]);
     }
   };
   if constexpr (PL) {
@@ -405,7 +398,6 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
   }
   T pre[PR];
   T pq[PQS];
-  if (!AHEAD && k0 + 2 <= k1) fetch(k0 + 2, pre);
   if constexpr (PL) {
     // phi plane k0+2 takes the slot of plane k0-2 once every projection
     // above is done
@@ -425,28 +417,20 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
     p_prev = ((const T*)s.p)[ob];
     if constexpr (PROJ) p_prev = p_prev + (phi[ob] - mv);
   }
-  // Order of the memory requests per plane k.
-  // AHEAD: the pointwise operands of plane k first, then the halo plane k+2
-  // into registers; the stencil work of plane k waits for the pointwise
-  // operands only (loads complete in issue order), so plane k+2 stays in
-  // flight during the work and is committed to LDS after it (256^3 FP64
-  // stage kernel: 356 -> 343 us).
-  // Otherwise: plane k+2 is committed and plane k+3 requested before the
-  // pointwise operands of plane k (FP32 on four slots: 175 us, against 181
-  // with the order above).
+  // Order of the memory requests per plane k: the pointwise operands of
+  // plane k first, then the halo plane k+2 into registers; the stencil work
+  // of plane k waits for the pointwise operands only (loads complete in issue
+  // order), so plane k+2 stays in flight during the work and is committed to
+  // LDS after it (256^3 FP64 stage kernel: 356 -> 343 us against committing
+  // plane k+2 and requesting plane k+3 before the pointwise operands).
   // The pointwise loads are unconditional: threads outside the tile load at
   // a clamped column (every address lies inside the allocation: i, j, k >= 1
   // and i <= imax, j <= jmax, k <= kmax) and their values are not used, so
   // the number of loads in flight is fixed for the compiler's waits.
   const long oc = L.idx(min(i, imax), min(j, jmax), 0);
   for (int k = k0; k < k1; k++) {
-    if constexpr (!AHEAD && NS >= 4) {
-      // the slot of plane k-2, which no thread reads any more
-      if (k + 2 <= k1) commit(k + 2, pre);
-      if (k + 3 <= k1) fetch(k + 3, pre);
-    }
     const long o = L.idx(i, j, k);
-    const long ol = AHEAD ? oc + (long)k * L.plane : o;
+    const long ol = oc + (long)k * L.plane;
     T p0 = (T)0, pm[3] = {(T)0, (T)0, (T)0}, av[3] = {(T)0, (T)0, (T)0}, bv[3] = {(T)0, (T)0, (T)0};
     T f0 = (T)0, fm[3] = {(T)0, (T)0, (T)0};  // fused projection: phi at the point and below it
     // pointwise operands of all three components, loaded together
@@ -473,28 +457,25 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
           }
       }
     };
-    if constexpr (AHEAD) {
+    {
       const bool every[3] = {true, true, true};
       load_pointwise(every);
     }
     // plane k+2 is read by plane k+1's stencil; past the chunk's last plane
     // the fetch repeats plane k1 (a cache hit) and is not committed
     const bool more = k + 2 <= k1;
-    if constexpr (AHEAD) {
-      fetch(min(k + 2, k1), pre);
-      // PL: phi plane k+3, which the projection of plane k+3 reads one plane
-      // later (and past the chunk's last plane a repeat, not committed)
-      pfetch(min(k + 3, k1), pq);
-    }
-    // AHEAD, the full step computes every component on every thread (the
-    // taps of a thread outside the field still lie in the LDS tile; only the
-    // stores are masked), so that the compiler keeps the loads above
-    // unconditional instead of sinking them into the branches
-    constexpr bool EVERY = AHEAD && KIND <= 3;
+    fetch(min(k + 2, k1), pre);
+    // PL: phi plane k+3, which the projection of plane k+3 reads one plane
+    // later (and past the chunk's last plane a repeat, not committed)
+    pfetch(min(k + 3, k1), pq);
+    // the full step computes every component on every thread (the taps of a
+    // thread outside the field still lie in the LDS tile; only the stores
+    // are masked), so that the compiler keeps the loads above unconditional
+    // instead of sinking them into the branches
+    constexpr bool EVERY = KIND <= 3;
     if (EVERY || in_tile) {
       bool inside[3];
       for (int c = 0; c < 3; c++) inside[c] = interior(g, c, i, j, k);
-      if constexpr (!AHEAD) load_pointwise(inside);
       // fused projection: the pressure after the previous stage's increment,
       // p + (phi - mean) (k_project's expression), at the point and below it
       T pc = p0;
@@ -588,23 +569,13 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
     // plane k+2 goes to the slot of plane k-1 (three slots: once every
     // thread is done with plane k) or of plane k-2 (four and more slots: no
     // thread has read it since the previous barrier)
-    if constexpr (AHEAD) {
-      if constexpr (NS == 3) __syncthreads();
-      if (more) commit(k + 2, pre);
-      // PL: phi plane k+3 into the slot of plane k-1 (the pressure carried
-      // from plane to plane leaves plane k-1 of phi unread here)
-      if constexpr (PL)
-        if (k + 3 <= k1) pcommit(k + 3, pq);
-      __syncthreads();
-    } else {
-      __syncthreads();
-      if constexpr (NS == 3) {
-        // plane k+2 replaces plane k-1; plane k+3 starts loading
-        if (more) commit(k + 2, pre);
-        if (k + 3 <= k1) fetch(k + 3, pre);
-        __syncthreads();
-      }
-    }
+    if constexpr (NS == 3) __syncthreads();
+    if (more) commit(k + 2, pre);
+    // PL: phi plane k+3 into the slot of plane k-1 (the pressure carried
+    // from plane to plane leaves plane k-1 of phi unread here)
+    if constexpr (PL)
+      if (k + 3 <= k1) pcommit(k + 3, pq);
+    __syncthreads();
   }
 }
 
