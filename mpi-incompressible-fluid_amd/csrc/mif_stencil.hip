@@ -157,9 +157,13 @@ constexpr bool kStageVec = true;
 template <typename T, int KIND, bool FT = false, bool NT = false, bool VEC = false, bool PROJ = false>
 // (FP64 full step: at most 80 VGPRs, i.e. three 512-thread workgroups per CU;
 // with the fused projection 128, two workgroups, which its LDS allows anyway;
-// FP32 with the fused projection: at most 80 VGPRs, three workgroups)
+// FP32 with the fused projection, and the FP32 velocity-only kinds with the
+// tabulated forcing: at most 80 VGPRs, three workgroups (256^3 mixed
+// velocity-only step 0.926 -> 0.908 ms; the FP64 ones spill under that cap,
+// 1.47 -> 1.86 ms))
 __global__ void __launch_bounds__(512)
-__attribute__((amdgpu_waves_per_eu(KIND <= 3 && (sizeof(T) == 8 || PROJ) ? (sizeof(T) == 8 && PROJ ? 4 : 6) : 1)))
+__attribute__((amdgpu_waves_per_eu(KIND <= 3 && (sizeof(T) == 8 || PROJ) ? (sizeof(T) == 8 && PROJ ? 4 : 6)
+                                                                      : (FT && sizeof(T) == 4 ? 6 : 1))))
 k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int kmax, int ntx, int nty, int ntiles) {
   static_assert(!PROJ || KIND <= 3, "a fused projection belongs to the full-step kinds");
   // ring slots: 3 (planes k-1, k, k+1; two barriers per plane) or 4 and
