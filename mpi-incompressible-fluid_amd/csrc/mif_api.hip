@@ -609,11 +609,12 @@ int pressure_solve_any(mif_ctx* c, mif_field* p, mif_field* const vel[3], double
 
 // phi (not null): the previous stage's projection fused in, with time step
 // pdts and the increment's mean c->phi_mean; the incremented pressure goes to
-// pout, here and (launch_p_rim) on the points the stage's tiles leave out.
+// pout, here and on the points the stage's tiles leave out: by the boundary
+// pass that follows (rim, for apply_bc), or by a pass of its own (rim null).
 template <typename T>
 int stage(mif_ctx* c, int st, mif_field* const X[3], mif_field* const A[3], mif_field* const O1[3],
           mif_field* const O2[3], const void* p, double a1, double a2, double b, const void* phi = nullptr,
-          double pdts = 0, void* pout = nullptr) {
+          double pdts = 0, void* pout = nullptr, PressureRim* rim = nullptr) {
   StageArgs s{};
   for (int q = 0; q < 3; q++) {
     s.X[q] = X[q]->data;
@@ -634,19 +635,23 @@ int stage(mif_ctx* c, int st, mif_field* const X[3], mif_field* const A[3], mif_
     KTimer t(c, MIF_KC_STAGE);
     launch_stage<T>(st, c->g, c->L, s, c->stream);
   }
-  if (phi) {
+  if (phi && rim) {
+    *rim = PressureRim{p, phi, pout, c->phi_mean};
+  } else if (phi) {
     KTimer t(c, MIF_KC_PRESSURE_RIM);
     launch_p_rim<T>(c->g, c->L, (const T*)p, (const T*)phi, (T*)pout, c->phi_mean, c->stream);
-    c->phi_mean = nullptr;
   }
+  if (phi) c->phi_mean = nullptr;
   return MIF_OK;
 }
 
+// rim: the pressure rim of the fused stage kernel just launched, written in
+// the same launch
 template <typename T>
-int apply_bc(mif_ctx* c, mif_field* const v[3], const FnDesc& f, double t) {
+int apply_bc(mif_ctx* c, mif_field* const v[3], const FnDesc& f, double t, const PressureRim& rim = PressureRim{}) {
   void* F[3] = {v[0]->data, v[1]->data, v[2]->data};
   KTimer tm(c, MIF_KC_BC);
-  launch_bc<T>(c->g, c->L, F, f, t, c->stream);
+  launch_bc<T>(c->g, c->L, F, f, t, c->stream, rim);
   return MIF_OK;
 }
 
@@ -731,12 +736,15 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
     c->sweeps = sweeps;
     return r;
   };
+  // the pressure rim of each fused stage kernel is written by the boundary
+  // pass that follows it (one launch)
+  PressureRim rim;
   // Stage 1 (src/Timestep.cpp:107-117), with the previous step's last
   // projection when a multi-step call left it pending
   {
     const double a1 = 64.0 / 120.0 * dt;
     if (fuse && cs.pending_dts > 0) {
-      rc = stage<T>(c, 1, vel, nullptr, buf, buf2, pin(), a1, 0.0, a1, pb->data, cs.pending_dts, pnext());
+      rc = stage<T>(c, 1, vel, nullptr, buf, buf2, pin(), a1, 0.0, a1, pb->data, cs.pending_dts, pnext(), &rim);
       cs.cur ^= 1;
       cs.pending_dts = 0;
     } else {
@@ -744,35 +752,35 @@ int full_timestep(mif_ctx* c, mif_field* const vel[3], mif_field* const buf[3], 
     }
     if (rc) return rc;
   }
-  apply_bc<T>(c, buf, ev[0], st.time[1]);
+  apply_bc<T>(c, buf, ev[0], st.time[1], rim);
   if ((rc = solve(1, buf))) return rc;
   if (!fuse) project<T>(c, buf, p, pb, st.dt_s[1]);
   // Stage 2 (src/Timestep.cpp:119-129)
   {
     const double a1 = -34.0 / 120.0 * dt, a2 = 50.0 / 120.0 * dt;
     if (fuse) {
-      rc = stage<T>(c, 2, buf, buf2, buf2, vel, pin(), a1, a2, a1 + a2, pb->data, st.dt_s[1], pnext());
+      rc = stage<T>(c, 2, buf, buf2, buf2, vel, pin(), a1, a2, a1 + a2, pb->data, st.dt_s[1], pnext(), &rim);
       cs.cur ^= 1;
     } else {
       rc = stage<T>(c, 2, buf, buf2, buf2, vel, pin(), a1, a2, a1 + a2);
     }
     if (rc) return rc;
   }
-  apply_bc<T>(c, buf2, ev[1], st.time[2]);
+  apply_bc<T>(c, buf2, ev[1], st.time[2], rim);
   if ((rc = solve(2, buf2))) return rc;
   if (!fuse) project<T>(c, buf2, p, pb, st.dt_s[2]);
   // Stage 3 (src/Timestep.cpp:131-140)
   {
     const double a2 = -50.0 / 120.0 * dt, a3 = 90.0 / 120.0 * dt;
     if (fuse) {
-      rc = stage<T>(c, 3, buf2, vel, vel, nullptr, pin(), 0.0, a3, a2 + a3, pb->data, st.dt_s[2], pnext());
+      rc = stage<T>(c, 3, buf2, vel, vel, nullptr, pin(), 0.0, a3, a2 + a3, pb->data, st.dt_s[2], pnext(), &rim);
       cs.cur ^= 1;
     } else {
       rc = stage<T>(c, 3, buf2, vel, vel, nullptr, pin(), 0.0, a3, a2 + a3);
     }
     if (rc) return rc;
   }
-  apply_bc<T>(c, vel, ev[2], st.time[3]);
+  apply_bc<T>(c, vel, ev[2], st.time[3], rim);
   if ((rc = solve(3, vel))) return rc;
   if (fuse && !last) {
     cs.pending_dts = st.dt_s[3];

@@ -98,7 +98,15 @@ struct FusedIO {
 template <typename T>
 void launch_stage(int stage, const DGrid& g, const Layout& L, const StageArgs& s, hipStream_t st);
 // pout = pin + (phi - mean) on the pressure points a stage kernel with a fused
-// projection leaves out (outside the union of the components' interiors)
+// projection leaves out (outside the union of the components' interiors):
+// on its own (launch_p_rim) or in the boundary pass that follows the stage
+// (launch_bc with a PressureRim)
+struct PressureRim {
+  const void* pin = nullptr;  // null: no rim
+  const void* phi = nullptr;
+  void* pout = nullptr;
+  const Red3* mean = nullptr;
+};
 template <typename T>
 void launch_p_rim(const DGrid& g, const Layout& L, const T* pin, const T* phi, T* pout, const Red3* mean,
                   hipStream_t st);
@@ -109,7 +117,8 @@ void launch_vel_stage(int stage, const DGrid& g, const Layout& L, const VelStage
 // (s = 1, the staggered axis of a component), i < tn.
 void launch_trig_tables(const DGrid& g, double* trig, int tn, hipStream_t st);
 template <typename T>
-void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& f, double t, hipStream_t st);
+void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& f, double t, hipStream_t st,
+               const PressureRim& rim = PressureRim{});
 template <typename T>
 void launch_periodic(const DGrid& g, const Layout& L, T* F, int stag, hipStream_t st);
 template <typename T>

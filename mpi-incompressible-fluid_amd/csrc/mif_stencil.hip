@@ -597,6 +597,71 @@ k_stage(DGrid g, Layout L, StageArgs s, VelStageArgs vs, int imax, int jmax, int
 // face (its value is a function evaluation), so every shell point is
 // computed independently and the pass needs no ordering between threads.
 // ---------------------------------------------------------------------------
+__host__ __device__ inline int rim_out_count(int n, int mx) { return 1 + max(0, n - 1 - mx); }
+__device__ inline int rim_out_index(int t, int mx) { return t == 0 ? 0 : mx + t; }
+
+// The extents of the stage kernels' tiles (the union of the components'
+// interiors) and the rim's point counts per group.
+struct RimShape {
+  int imax, jmax, kmax;
+  long count[3];
+};
+inline RimShape rim_shape(const DGrid& g) {
+  int eu[3], ev[3], ew[3];
+  extents(g, 0, eu);
+  extents(g, 1, ev);
+  extents(g, 2, ew);
+  RimShape r;
+  r.imax = max(eu[0], max(ev[0], ew[0])) - 2;
+  r.jmax = max(eu[1], max(ev[1], ew[1])) - 2;
+  r.kmax = max(eu[2], max(ev[2], ew[2])) - 2;
+  const int n0 = g.N[0], n1 = g.N[1], n2 = g.N[2];
+  r.count[0] = (long)rim_out_count(n2, r.kmax) * n0 * n1;
+  r.count[1] = (long)rim_out_count(n1, r.jmax) * n0 * min(r.kmax, n2 - 1);
+  r.count[2] = (long)rim_out_count(n0, r.imax) * min(r.jmax, n1 - 1) * min(r.kmax, n2 - 1);
+  return r;
+}
+
+// Point q of rim group grp: pout = pin + (phi - mean).
+template <typename T>
+__device__ inline void p_rim_point(const DGrid& g, const Layout& L, int grp, long q, const T* __restrict__ pin,
+                                   const T* __restrict__ phi, T* __restrict__ pout, const Red3* __restrict__ mean,
+                                   int imax, int jmax, int kmax) {
+  const int n0 = g.N[0], n1 = g.N[1], n2 = g.N[2];
+  const int ki = min(kmax, n2 - 1), ji = min(jmax, n1 - 1);  // inner ranges [1, ki], [1, ji]
+  int i, j, k;
+  if (grp == 0) {
+    if (q >= (long)rim_out_count(n2, kmax) * n0 * n1) return;
+    i = (int)(q % n0);
+    j = (int)((q / n0) % n1);
+    k = rim_out_index((int)(q / ((long)n0 * n1)), kmax);
+  } else if (grp == 1) {
+    if (q >= (long)rim_out_count(n1, jmax) * n0 * ki) return;
+    i = (int)(q % n0);
+    const long r = q / n0;
+    j = rim_out_index((int)(r % rim_out_count(n1, jmax)), jmax);
+    k = 1 + (int)(r / rim_out_count(n1, jmax));
+  } else {
+    const int no = rim_out_count(n0, imax);
+    if (q >= (long)no * ji * ki) return;
+    i = rim_out_index((int)(q % no), imax);
+    const long r = q / no;
+    j = 1 + (int)(r % ji);
+    k = 1 + (int)(r / ji);
+  }
+  const T mv = mean ? (T)(mean->s1 / mean->s2) : (T)0;
+  const long o = L.idx(i, j, k);
+  pout[o] = pin[o] + (phi[o] - mv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_p_rim(DGrid g, Layout L, const T* __restrict__ pin, const T* __restrict__ phi,
+                                               T* __restrict__ pout, const Red3* __restrict__ mean, int imax, int jmax,
+                                               int kmax) {
+  p_rim_point<T>(g, L, blockIdx.y, (long)blockIdx.x * blockDim.x + threadIdx.x, pin, phi, pout, mean, imax, jmax,
+                 kmax);
+}
+
 __device__ inline double face_value(const DGrid& g, const FnDesc& f, double t, int c, int face, int i, int j, int k) {
   // faces 0,1 = z-,z+; 2,3 = y-,y+; 4,5 = x-,x+
   const int axis = 2 - (face >> 1);
@@ -683,8 +748,17 @@ __device__ inline T shell_value(const DGrid& g, const Layout& L, const BCArgs& a
 // two points is the other's source-side neighbour (ghost 0 and point 1 share
 // a row segment, as do e-2 and ghost e-1), so both reads come first and the
 // two writes then land on lines the same thread has just read.
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a) {
+// RIM: blocks with blockIdx.y >= 9 write the pressure rim of the fused
+// stage kernel before this pass (PressureRim), in the same launch.
+template <typename T, int MODE, bool RIM = false>
+__global__ void __launch_bounds__(256) k_bc(DGrid g, Layout L, BCArgs a, PressureRim rim, RimShape rs) {
+  if constexpr (RIM) {
+    if (blockIdx.y >= 9) {
+      p_rim_point<T>(g, L, blockIdx.y - 9, (long)blockIdx.x * blockDim.x + threadIdx.x, (const T*)rim.pin,
+                     (const T*)rim.phi, (T*)rim.pout, rim.mean, rs.imax, rs.jmax, rs.kmax);
+      return;
+    }
+  }
   const int c = blockIdx.y / 3, pair = blockIdx.y % 3;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.count[c][pair]) return;
@@ -1143,58 +1217,13 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
 // Three groups of points: the k planes outside [1, kmax]; within it the j
 // rows outside [1, jmax]; within both the i columns outside [1, imax].
 // ---------------------------------------------------------------------------
-__host__ __device__ inline int rim_out_count(int n, int mx) { return 1 + max(0, n - 1 - mx); }
-__device__ inline int rim_out_index(int t, int mx) { return t == 0 ? 0 : mx + t; }
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_p_rim(DGrid g, Layout L, const T* __restrict__ pin, const T* __restrict__ phi,
-                                               T* __restrict__ pout, const Red3* __restrict__ mean, int imax, int jmax,
-                                               int kmax) {
-  const int grp = blockIdx.y;
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int n0 = g.N[0], n1 = g.N[1], n2 = g.N[2];
-  const int ki = min(kmax, n2 - 1), ji = min(jmax, n1 - 1);  // inner ranges [1, ki], [1, ji]
-  int i, j, k;
-  if (grp == 0) {
-    if (q >= (long)rim_out_count(n2, kmax) * n0 * n1) return;
-    i = (int)(q % n0);
-    j = (int)((q / n0) % n1);
-    k = rim_out_index((int)(q / ((long)n0 * n1)), kmax);
-  } else if (grp == 1) {
-    if (q >= (long)rim_out_count(n1, jmax) * n0 * ki) return;
-    i = (int)(q % n0);
-    const long r = q / n0;
-    j = rim_out_index((int)(r % rim_out_count(n1, jmax)), jmax);
-    k = 1 + (int)(r / rim_out_count(n1, jmax));
-  } else {
-    const int no = rim_out_count(n0, imax);
-    if (q >= (long)no * ji * ki) return;
-    i = rim_out_index((int)(q % no), imax);
-    const long r = q / no;
-    j = 1 + (int)(r % ji);
-    k = 1 + (int)(r / ji);
-  }
-  const T mv = mean ? (T)(mean->s1 / mean->s2) : (T)0;
-  const long o = L.idx(i, j, k);
-  pout[o] = pin[o] + (phi[o] - mv);
-}
-
 template <typename T>
 void launch_p_rim(const DGrid& g, const Layout& L, const T* pin, const T* phi, T* pout, const Red3* mean,
                   hipStream_t st) {
-  int eu[3], ev[3], ew[3];
-  extents(g, 0, eu);
-  extents(g, 1, ev);
-  extents(g, 2, ew);
-  const int imax = max(eu[0], max(ev[0], ew[0])) - 2;
-  const int jmax = max(eu[1], max(ev[1], ew[1])) - 2;
-  const int kmax = max(eu[2], max(ev[2], ew[2])) - 2;
-  const int n0 = g.N[0], n1 = g.N[1], n2 = g.N[2];
-  const long c0 = (long)rim_out_count(n2, kmax) * n0 * n1;
-  const long c1 = (long)rim_out_count(n1, jmax) * n0 * min(kmax, n2 - 1);
-  const long c2 = (long)rim_out_count(n0, imax) * min(jmax, n1 - 1) * min(kmax, n2 - 1);
-  const long mx = max(c0, max(c1, c2));
-  k_p_rim<T><<<dim3((unsigned)((mx + 255) / 256), 3, 1), 256, 0, st>>>(g, L, pin, phi, pout, mean, imax, jmax, kmax);
+  const RimShape rs = rim_shape(g);
+  const long mx = max(rs.count[0], max(rs.count[1], rs.count[2]));
+  k_p_rim<T><<<dim3((unsigned)((mx + 255) / 256), 3, 1), 256, 0, st>>>(g, L, pin, phi, pout, mean, rs.imax, rs.jmax,
+                                                                      rs.kmax);
 }
 
 template <typename T>
@@ -1242,7 +1271,8 @@ static void shell_counts(const DGrid& g, int stag, int cnt[3]) {
 }
 
 template <typename T>
-void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& f, double t, hipStream_t st) {
+void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& f, double t, hipStream_t st,
+               const PressureRim& rim) {
   BCArgs a{};
   int mx = 0;
   for (int c = 0; c < 3; c++) {
@@ -1256,13 +1286,27 @@ void launch_bc(const DGrid& g, const Layout& L, void* const F[3], const FnDesc& 
   }
   a.f = f;
   a.t = t;
+  const RimShape rs = rim_shape(g);
+  if (rim.pin) {
+    // three more block rows for the pressure rim, as many blocks as either needs
+    const long rmx = max(rs.count[0], max(rs.count[1], rs.count[2]));
+    const unsigned nbx = (unsigned)max((long)(mx + 255) / 256, (rmx + 255) / 256);
+    const dim3 grd(nbx, 12, 1);
+    if (f.kind == MIF_FN_ZERO)
+      k_bc<T, BC_ZERO, true><<<grd, 256, 0, st>>>(g, L, a, rim, rs);
+    else if (f.kind == kFnFaceTable)
+      k_bc<T, BC_TABLE, true><<<grd, 256, 0, st>>>(g, L, a, rim, rs);
+    else
+      k_bc<T, BC_FN, true><<<grd, 256, 0, st>>>(g, L, a, rim, rs);
+    return;
+  }
   dim3 grd((mx + 255) / 256, 9, 1);
   if (f.kind == MIF_FN_ZERO)
-    k_bc<T, BC_ZERO><<<grd, 256, 0, st>>>(g, L, a);
+    k_bc<T, BC_ZERO><<<grd, 256, 0, st>>>(g, L, a, rim, rs);
   else if (f.kind == kFnFaceTable)
-    k_bc<T, BC_TABLE><<<grd, 256, 0, st>>>(g, L, a);
+    k_bc<T, BC_TABLE><<<grd, 256, 0, st>>>(g, L, a, rim, rs);
   else
-    k_bc<T, BC_FN><<<grd, 256, 0, st>>>(g, L, a);
+    k_bc<T, BC_FN><<<grd, 256, 0, st>>>(g, L, a, rim, rs);
 }
 
 template <typename T>
@@ -1395,7 +1439,8 @@ void launch_gather(const Layout& L, const T* F, const int* ijk, long n, double* 
   template void launch_stage<T>(int, const DGrid&, const Layout&, const StageArgs&, hipStream_t);                   \
   template void launch_p_rim<T>(const DGrid&, const Layout&, const T*, const T*, T*, const Red3*, hipStream_t);    \
   template void launch_vel_stage<T>(int, const DGrid&, const Layout&, const VelStageArgs&, hipStream_t);            \
-  template void launch_bc<T>(const DGrid&, const Layout&, void* const*, const FnDesc&, double, hipStream_t);         \
+  template void launch_bc<T>(const DGrid&, const Layout&, void* const*, const FnDesc&, double, hipStream_t,         \
+                             const PressureRim&);                                                                   \
   template void launch_periodic<T>(const DGrid&, const Layout&, T*, int, hipStream_t);                              \
   template void launch_set<T>(const DGrid&, const Layout&, T*, int, int, const FnDesc&, double, int, hipStream_t);  \
   template void launch_project<T>(const DGrid&, const Layout&, T*, T*, T*, T*, const T*, double, hipStream_t,       \
