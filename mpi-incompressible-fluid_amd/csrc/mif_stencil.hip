@@ -1192,9 +1192,12 @@ static void launch_stage_kind(const DGrid& g, const Layout& L, const StageArgs& 
   // 1.359 ms, 128^3 FP64 0.523 -> 0.509 ms, 192^3 mixed 0.787 -> 0.775 ms;
   // the 256^3 mixed stage kernel (non-temporal stores) 181 -> 185 us (182
   // with the LDS ring padded to six slots, i.e. three workgroups per CU as
-  // with scalar loads), so FP32 with non-temporal stores keeps the scalar
-  // loads (on the FP32 128 x 4 tiles: 176.2 -> 175.9 us, no gain either).
-  if constexpr (kStageVec && KIND <= 3 && (sizeof(T) == 8 || !NT)) {
+  // with scalar loads), so FP32 with non-temporal stores kept the scalar
+  // loads (on the FP32 128 x 4 tiles: 176.2 -> 175.9 us, no gain either;
+  // the FP32 kernels now take field-major slots, which ignore VEC).  The
+  // FP64 velocity-only kinds take them too: 256^3-node step 1.467 -> 1.432
+  // ms.
+  if constexpr (kStageVec && (KIND <= 3 || sizeof(T) == 8) && (sizeof(T) == 8 || !NT)) {
     if (L.lead * (long)sizeof(T) == 128 - (long)sizeof(T) && (L.PX * sizeof(T)) % 16 == 0) {
       if constexpr (kFuse) {
         if (s.phi) return go(yes{}, yes{});
