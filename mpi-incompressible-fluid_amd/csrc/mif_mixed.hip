@@ -154,30 +154,49 @@ __global__ void __launch_bounds__(256) k_accumulate(DGrid g, double* __restrict_
   P[o] += (double)B32[o];
 }
 
-__global__ void __launch_bounds__(256) k_reduce_sum2(const Red3* partial, int n, Red3* out) {
-  __shared__ double sh[2][256];
-  double s = 0, sw = 0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    s += partial[q].s1;
-    sw += partial[q].s2;
-  }
-  sh[0][threadIdx.x] = s;
-  sh[1][threadIdx.x] = sw;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + st];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + st];
+// (s1, s2) summed over n per-workgroup partials by one workgroup of
+// kReduceThreads threads, each with four independent accumulations in flight
+// (256^3: 8192 partials; 256 threads with one chain each took 12 us)
+constexpr int kReduceThreads = 1024;
+__global__ void __launch_bounds__(kReduceThreads) k_reduce_sum2(const Red3* partial, int n, Red3* out) {
+  __shared__ double sh[2][kReduceThreads / 64];
+  double s[4] = {0, 0, 0, 0}, sw[4] = {0, 0, 0, 0};
+  int q = threadIdx.x;
+  for (; q + 3 * kReduceThreads < n; q += 4 * kReduceThreads) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      s[u] += partial[q + u * kReduceThreads].s1;
+      sw[u] += partial[q + u * kReduceThreads].s2;
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) *out = Red3{sh[0][0], sh[1][0], 0};
+  for (; q < n; q += kReduceThreads) {
+    s[0] += partial[q].s1;
+    sw[0] += partial[q].s2;
+  }
+  double a = (s[0] + s[1]) + (s[2] + s[3]), b = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0, tb = 0;
+    for (int w = 0; w < kReduceThreads / 64; w++) {
+      ta += sh[0][w];
+      tb += sh[1][w];
+    }
+    *out = Red3{ta, tb, 0};
+  }
 }
 
 void launch_project_rhs(const DGrid& g, double* B64, float* B32, long bp0, long bplane, Red3* partial, int nblocks,
                         Red3* total, hipStream_t st) {
   k_wsum<<<nblocks, 256, 0, st>>>(g, B64, bp0, bplane, partial);
-  k_reduce_sum2<<<1, 256, 0, st>>>(partial, nblocks, total);
+  k_reduce_sum2<<<1, kReduceThreads, 0, st>>>(partial, nblocks, total);
   dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
   k_sub_mean_to32<<<grd, 256, 0, st>>>(g, B64, B32, bp0, bplane, total);
 }
@@ -187,7 +206,7 @@ void launch_project_rhs(const DGrid& g, double* B64, float* B32, long bp0, long 
 void launch_remove_mean(const DGrid& g, double* P, long bp0, long bplane, Red3* partial, int nblocks, Red3* total,
                         hipStream_t st) {
   k_wsum<<<nblocks, 256, 0, st>>>(g, P, bp0, bplane, partial);
-  k_reduce_sum2<<<1, 256, 0, st>>>(partial, nblocks, total);
+  k_reduce_sum2<<<1, kReduceThreads, 0, st>>>(partial, nblocks, total);
   dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
   k_sub_mean_to32<<<grd, 256, 0, st>>>(g, P, nullptr, bp0, bplane, total);
 }
@@ -210,7 +229,7 @@ void launch_residual32(const DGrid& g, const double* b, const float* P, const Re
 }
 
 void launch_reduce_partials(const Red3* partial, int n, Red3* total, hipStream_t st) {
-  k_reduce_sum2<<<1, 256, 0, st>>>(partial, n, total);
+  k_reduce_sum2<<<1, kReduceThreads, 0, st>>>(partial, n, total);
 }
 
 void launch_sub_mean_field(const DGrid& g, const Layout& L, float* F, const Red3* total, hipStream_t st) {
