@@ -526,9 +526,9 @@ int pressure_solve_mixed64(mif_ctx* c, mif_field* p, mif_field* const vel[3], do
 //   x forward + divergence: b = div(u)/dt in FP64, stored (b64) with its
 //     weighted sums (W.b, W.1), and its x transform in FP32;
 //   y, z (scale), y, x inverse: phi_0 in FP32 (B);
-//   residual: r = (b - mean_b) - L phi_0 in FP64 (FP32 phi_0 read exactly),
-//     stored in FP32 (B2);
-//   x, y, z, y transforms of r; x inverse + unpack: phi = phi_0 + delta in
+//   x forward of the defect r = (b - mean_b) - L phi_0, formed in FP64 as
+//     the pass loads its lines (FP32 phi_0 read exactly), into B2;
+//   y, z, y transforms of r; x inverse + unpack: phi = phi_0 + delta in
 //     FP64, stored as the FP32 increment with its periodic images, and its
 //     weighted sums (W.phi, W.1).
 // The increment's weighted mean (FP32 rounding of the zeroed constant mode)
@@ -556,12 +556,14 @@ int pressure_solve_mixed_dc1(mif_ctx* c, mif_field* p, mif_field* const vel[3], 
     launch_fft_axis<float>(c->g, 0, +1, 0, c->plan[0], sc, B32, c->bp0, c->bplane, c->stream);
   }
   {
-    KTimer t(c, MIF_KC_DEFECT);
-    launch_residual32(c->g, c->B64, B32, c->red_b, R32, c->bp0, c->bplane, c->stream);
-  }
-  {
+    // the x forward pass forms the defect as it loads its lines
     KTimer t(c, MIF_KC_FFT_X);
-    launch_fft_axis<float>(c->g, 0, -1, 0, c->plan[0], sc, R32, c->bp0, c->bplane, c->stream);
+    FusedIO rio{};
+    rio.res = 1;
+    rio.phi32 = B32;
+    rio.b64 = c->B64;
+    rio.bmean = c->red_b;
+    launch_fft_pow2<float>(c->g, 0, 0, c->plan[0], sc, R32, c->bp0, c->bplane, &rio, c->stream);
   }
   spectral_middle<float>(c, R32);
   io.add32 = B32;

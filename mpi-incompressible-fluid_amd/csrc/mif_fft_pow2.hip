@@ -216,10 +216,14 @@ constexpr int kTwlMax = 65536;
 // TV: element type of the velocity and pressure fields of the fused passes.
 // DC (fused x passes of the mixed form's one-sweep defect correction, FP32):
 // see FusedIO::dc.
-template <typename T, int L, int NP, int AXIS, int MODE, bool FUSED, typename TV = T, bool DC = false>
+// RES (x forward pass, FP32): the lines are the defect r = (b - mean) - L phi
+// of the first FP32 solution (FusedIO::res), formed in FP64 as they load.
+template <typename T, int L, int NP, int AXIS, int MODE, bool FUSED, typename TV = T, bool DC = false,
+          bool RES = false>
 __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W, SpectralScale sc, T* __restrict__ B,
                                                 long bp0, long bplane, int periodic, FusedIO io) {
   static_assert(!DC || (FUSED && AXIS == 0), "defect-correction outputs belong to the fused x passes");
+  static_assert(!RES || (AXIS == 0 && MODE == 0 && !FUSED), "the defect is formed by the x forward pass");
   // twiddles live in LDS when the workgroup stays within kTwlMax
   // bytes, otherwise they are read from the (L1-resident) global table
   constexpr int LP = p2::lds_pitch(L);
@@ -282,6 +286,16 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
           if (js[qj] >= 0 && ks[qk] >= 0) rows[2 * qk + qj] = io.L.idx(0, js[qj], ks[qk]);
     }
     for (int q = 0; q < 4; q++) prow[t][q] = rows[q];
+  }
+  // defect lines (RES): the rows of each line's y and z neighbours
+  __shared__ long yzrow[RES ? 2 * NP : 1][4];
+  if (RES && t < NL) {
+    const int r = bx * NL + t;
+    const int j = r < n1 * n2 ? r % n1 : 0, k = r < n1 * n2 ? r / n1 : 0;
+    yzrow[t][0] = bp0 * nb(g, 1, j, -1) + bplane * k;
+    yzrow[t][1] = bp0 * nb(g, 1, j, 1) + bplane * k;
+    yzrow[t][2] = bp0 * j + bplane * nb(g, 2, k, -1);
+    yzrow[t][3] = bp0 * j + bplane * nb(g, 2, k, 1);
   }
   if (t < NL) {
     long b = -1;
@@ -359,12 +373,48 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   } else if (AXIS == 0) {
     constexpr int CL0 = R0 >= 16 ? 1 : 16 / R0;  // lines per chunk: 16 values in flight
     constexpr int CL = CL0 < NL ? CL0 : NL;
+    // RES: b's weighted mean and the 7-point operator's coefficients
+    const double rmean = RES ? io.bmean->s1 / io.bmean->s2 : 0.0;
+    const double ihx2 = 1.0 / (g.h[0] * g.h[0]), ihy2 = 1.0 / (g.h[1] * g.h[1]), ihz2 = 1.0 / (g.h[2] * g.h[2]);
+    // RES: each thread's points and their x neighbours in a line (the same
+    // for every line; points past the line read point 0)
+    int xo[RES ? R0 : 1], xm[RES ? R0 : 1], xp[RES ? R0 : 1];
+    if constexpr (RES) {
+#pragma unroll
+      for (int q = 0; q < R0; q++) {
+        const int m = t + 256 * q;
+        xo[q] = m < n ? m : 0;
+        xm[q] = (int)nb(g, 0, xo[q], -1);
+        xp[q] = (int)nb(g, 0, xo[q], 1);
+      }
+    }
     for (int l0 = 0; l0 < NL; l0 += CL) {
       T val[CL][R0];
 #pragma unroll
       for (int a = 0; a < CL; a++) {
         const int l = l0 + a;
         const long b = l < NL ? lbase[l] : -1;
+        if constexpr (RES) {
+          // r = (b - mean) - (lx + ly + lz) in FP64, the 7-point operator
+          // scaled by 1/h^2; a line outside the grid reads row 0 and is dropped
+          const float* P = io.phi32;
+          const int lc = l < NL ? l : 0;
+          const long row = b >= 0 ? b : 0;
+          const long rym = yzrow[lc][0], ryp = yzrow[lc][1], rzm = yzrow[lc][2], rzp = yzrow[lc][3];
+#pragma unroll
+          for (int q = 0; q < R0; q++) {
+            const int m = t + 256 * q;
+            const bool okm = b >= 0 && m < n;
+            const long o = row + xo[q];
+            const double c = P[o];
+            const double lx = ((double)P[row + xm[q]] - 2 * c + (double)P[row + xp[q]]) * ihx2;
+            const double ly = ((double)P[rym + xo[q]] - 2 * c + (double)P[ryp + xo[q]]) * ihy2;
+            const double lz = ((double)P[rzm + xo[q]] - 2 * c + (double)P[rzp + xo[q]]) * ihz2;
+            const T x = (T)((io.b64[o] - rmean) - (lx + ly + lz));
+            val[a][q] = okm ? x : (T)0;
+          }
+          continue;
+        }
 #pragma unroll
         for (int q = 0; q < R0; q++) {
           const int m = t + 256 * q;
@@ -992,7 +1042,10 @@ static void launch_p2(const DGrid& g, int axis, int mode, const FftPlan& plan, c
   const int per = plan.periodic;
 #define MIF_P2(AX, MD) k_fft_p2<T, L, NP, AX, MD, false><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, none)
   if constexpr (PASSES & 1) {
-    if (axis == 0 && io && io->dc) {
+    if (axis == 0 && io && io->res) {
+      if constexpr (sizeof(T) == 4)
+        k_fft_p2<T, L, NP, 0, 0, false, T, false, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);
+    } else if (axis == 0 && io && io->dc) {
       if constexpr (sizeof(T) == 4) {
         if (mode == 0)
           k_fft_p2<T, L, NP, 0, 0, true, T, true><<<grd, 256, 0, st>>>(g, W, sc, B, bp0, bplane, per, *io);

@@ -72,6 +72,25 @@ struct SpectralScale {
   double inv_norm;
 };
 
+// Neighbour index along axis a of packed index i (d = -1 or +1): periodic
+// wrap, cell-centred reflection of the first/last cell, or the node-centred
+// even reflection (the 7-point operator of the packed solve).
+__device__ inline long nb(const DGrid& g, int a, int i, int d) {
+  const int n = g.npts[a];
+  int q = i + d;
+  if (g.periodic[a]) {
+    q = q < 0 ? q + n : (q >= n ? q - n : q);
+  } else if (g.cell[a]) {
+    // cell-centred: the ghost mirrors the first/last cell
+    if (q < 0) q = 0;
+    if (q >= n) q = n - 1;
+  } else {
+    if (q < 0) q = n > 1 ? 1 : 0;
+    if (q >= n) q = n > 1 ? n - 2 : 0;
+  }
+  return q;
+}
+
 // Operands of the x passes fused with the divergence (forward) or with the
 // unpack into the pressure field (inverse).
 struct FusedIO {
@@ -93,6 +112,12 @@ struct FusedIO {
   double* b64 = nullptr;
   const float* add32 = nullptr;
   Red3* partial = nullptr;
+  // x forward pass of the defect (res): the pass loads r = (b - mean) - L phi
+  // in FP64 (b64, bmean, the first FP32 solution phi32, packed layout) instead
+  // of its buffer, and transforms it into its buffer
+  int res = 0;
+  const float* phi32 = nullptr;
+  const Red3* bmean = nullptr;
 };
 
 template <typename T>
@@ -223,11 +248,8 @@ void launch_remove_mean(const DGrid& g, double* P, long bp0, long bplane, Red3* 
 void launch_residual(const DGrid& g, const double* b, const double* P, float* B32, long bp0, long bplane,
                      hipStream_t st);
 void launch_accumulate(const DGrid& g, double* P, const float* B32, long bp0, long bplane, hipStream_t st);
-// One-sweep defect correction with FP32 transforms: R32 = (b - mean) - L phi_0
-// (FP64 arithmetic, phi_0 the FP32 first solution); the sum of n partial
+// One-sweep defect correction with FP32 transforms: the sum of n partial
 // (W.x, W.1) pairs into total; a node field minus the mean total holds.
-void launch_residual32(const DGrid& g, const double* b, const float* P, const Red3* total, float* R32, long bp0,
-                       long bplane, hipStream_t st);
 void launch_reduce_partials(const Red3* partial, int n, Red3* total, hipStream_t st);
 void launch_sub_mean_field(const DGrid& g, const Layout& L, float* F, const Red3* total, hipStream_t st);
 

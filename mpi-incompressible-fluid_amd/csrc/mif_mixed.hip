@@ -77,23 +77,6 @@ __global__ void __launch_bounds__(256) k_to64(DGrid g, const float* __restrict__
   P[o] = (double)B32[o];
 }
 
-__device__ inline long nb(const DGrid& g, int a, int i, int d) {
-  // neighbour index along axis a of packed index i (d = -1 or +1)
-  const int n = g.npts[a];
-  int q = i + d;
-  if (g.periodic[a]) {
-    q = q < 0 ? q + n : (q >= n ? q - n : q);
-  } else if (g.cell[a]) {
-    // cell-centred: the ghost mirrors the first/last cell
-    if (q < 0) q = 0;
-    if (q >= n) q = n - 1;
-  } else {
-    if (q < 0) q = n > 1 ? 1 : 0;
-    if (q >= n) q = n > 1 ? n - 2 : 0;
-  }
-  return q;
-}
-
 // B32 = (float)(b - L phi) in FP64.
 __global__ void __launch_bounds__(256) k_residual(DGrid g, const double* __restrict__ b, const double* __restrict__ P,
                                                   float* __restrict__ B32, long bp0, long bplane) {
@@ -109,31 +92,6 @@ __global__ void __launch_bounds__(256) k_residual(DGrid g, const double* __restr
   const double lz = (P[i + bp0 * j + bplane * nb(g, 2, k, -1)] - 2 * c + P[i + bp0 * j + bplane * nb(g, 2, k, 1)]) /
                     (g.h[2] * g.h[2]);
   B32[o] = (float)(b[o] - (lx + ly + lz));
-}
-
-// One-sweep defect correction (FP32 transforms): R32 = (float)((b - mean_b)
-// - L phi_0) in FP64, phi_0 read from the FP32 first solution; mean_b =
-// total->s1 / total->s2, the owner-weighted mean of b (the projection of the
-// right-hand side onto the range of the operator).
-__global__ void __launch_bounds__(256) k_residual32(DGrid g, const double* __restrict__ b, const float* __restrict__ P,
-                                                    const Red3* __restrict__ total, float* __restrict__ R32, long bp0,
-                                                    long bplane) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.npts[0]) return;
-  const long o = i + bp0 * j + bplane * k;
-  const double mean = total->s1 / total->s2;
-  const double c = P[o];
-  const double lx = ((double)P[nb(g, 0, i, -1) + bp0 * j + bplane * k] - 2 * c +
-                     (double)P[nb(g, 0, i, 1) + bp0 * j + bplane * k]) /
-                    (g.h[0] * g.h[0]);
-  const double ly = ((double)P[i + bp0 * nb(g, 1, j, -1) + bplane * k] - 2 * c +
-                     (double)P[i + bp0 * nb(g, 1, j, 1) + bplane * k]) /
-                    (g.h[1] * g.h[1]);
-  const double lz = ((double)P[i + bp0 * j + bplane * nb(g, 2, k, -1)] - 2 * c +
-                     (double)P[i + bp0 * j + bplane * nb(g, 2, k, 1)]) /
-                    (g.h[2] * g.h[2]);
-  R32[o] = (float)((b[o] - mean) - (lx + ly + lz));
 }
 
 // Subtract total->s1 / total->s2 from every point of a node field.
@@ -222,11 +180,6 @@ void launch_residual(const DGrid& g, const double* b, const double* P, float* B3
   k_residual<<<grd, 256, 0, st>>>(g, b, P, B32, bp0, bplane);
 }
 
-void launch_residual32(const DGrid& g, const double* b, const float* P, const Red3* total, float* R32, long bp0,
-                       long bplane, hipStream_t st) {
-  dim3 grd((g.npts[0] + 255) / 256, g.npts[1], g.npts[2]);
-  k_residual32<<<grd, 256, 0, st>>>(g, b, P, total, R32, bp0, bplane);
-}
 
 void launch_reduce_partials(const Red3* partial, int n, Red3* total, hipStream_t st) {
   k_reduce_sum2<<<1, kReduceThreads, 0, st>>>(partial, n, total);

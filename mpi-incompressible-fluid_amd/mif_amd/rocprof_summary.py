@@ -36,12 +36,13 @@ ALG_WORDS = [
     # words of 4 bytes, an FP64 array counting two)
     (r"k_fft_p2<float, \d+, \d+, 0, 0, true, float, true>", 6.0, "x forward + divergence: u,v,w -> B, b64"),
     (r"k_fft_p2<float, \d+, \d+, 0, 1, true, float, true>", 3.0, "x inverse + unpack: B, B32 -> phi"),
+    (r"k_fft_p2<float, \d+, \d+, 0, 0, false, float, false, true>", 4.0,
+     "x forward of the defect: phi32 (7-point), b64 -> B"),
     (r"k_fft_p2<[^>]*, 0, 0, true(, \w+)*>", 4.0, "x forward + divergence: u,v,w -> B"),
     (r"k_fft_p2<[^>]*, 0, 1, true(, \w+)*>", 2.0, "x inverse + unpack: B -> phi"),
     (r"k_fft_p2<[^>]*, 0, [01], false(, \w+)*>", 2.0, "x pass: B -> B"),
     (r"k_fft_p2<[^>]*, 1, [01], false(, \w+)*>", 2.0, "y pass: B -> B"),
     (r"k_fft_p2<[^>]*, 2, 2, false(, \w+)*>", 2.0, "z forward/scale/inverse: B -> B"),
-    (r"k_residual32\b", 4.0, "phi32 (7-point), b64 -> r32 (4-byte words)"),
     (r"k_project<", 9.0, "u,v,w,phi,p -> u,v,w,p"),
 ]
 
@@ -76,8 +77,6 @@ def _counter(path: str, ctr: str):
 
 
 def element_size(name: str) -> int:
-    if re.search(r"k_residual32\b", name):
-        return 4
     m = re.search(r"<(float|double)", name)
     return 4 if m and m.group(1) == "float" else 8
 
