@@ -34,8 +34,8 @@ ALG_WORDS = [
     (r"k_stage<[^,]+, 3(, \w+)*>", 10.0, "stage 3: Y3(3),S(3),p -> u(3)"),
     # the fused defect-correction x passes of the mixed form (FP32 kernels;
     # words of 4 bytes, an FP64 array counting two)
-    (r"k_fft_p2<float, \d+, \d+, 0, 0, true, float, true>", 6.0, "x forward + divergence: u,v,w -> B, b64"),
-    (r"k_fft_p2<float, \d+, \d+, 0, 1, true, float, true>", 3.0, "x inverse + unpack: B, B32 -> phi"),
+    (r"k_fft_p2<float, \d+, \d+, 0, 0, true, float, true(, false)?>", 6.0, "x forward + divergence: u,v,w -> B, b64"),
+    (r"k_fft_p2<float, \d+, \d+, 0, 1, true, float, true(, false)?>", 3.0, "x inverse + unpack: B, B32 -> phi"),
     (r"k_fft_p2<float, \d+, \d+, 0, 0, false, float, false, true>", 4.0,
      "x forward of the defect: phi32 (7-point), b64 -> B"),
     (r"k_fft_p2<[^>]*, 0, 0, true(, \w+)*>", 4.0, "x forward + divergence: u,v,w -> B"),
