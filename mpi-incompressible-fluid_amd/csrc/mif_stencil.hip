@@ -135,8 +135,12 @@ struct StageTile {
 // order above).
 // The FP64 kernels with a fused projection take three ring slots: with the
 // LDS phi ring (71 KB of LDS, two workgroups per CU) a fourth would leave one.
+// Their chunk is 12 planes (256^3 stage kernels: 6 planes 381 us, 8 377,
+// 12 372, 16 383); the FP32 ones keep 16 (8 planes 200 us, 16 196, 24 199).
 template <typename T, int KIND, bool NT, bool PROJ = false>
-constexpr int stage_planes() { return sizeof(T) == 8 && (NT || KIND >= 4 || PROJ) ? 8 : 16; }
+constexpr int stage_planes() {
+  return sizeof(T) == 8 && PROJ ? 12 : (sizeof(T) == 8 && (NT || KIND >= 4) ? 8 : 16);
+}
 constexpr int kStageSlotsFused64 = 3;
 
 // ring slots of the FP32 and FP64 stage kernels (see k_stage)
