@@ -86,10 +86,14 @@ def test_fused_matches_unfused_mixed(sweeps):
     assert rel_err(fp, up) < 1e-4
 
 
-def _run_n(gs, init, nsteps, fn_kind, precision=None, records=False, multi=True):
+def _run_n(gs, init, nsteps, fn_kind, precision=None, records=False, multi=True, sweeps=None, schedule=None):
     kw = {} if precision is None else {"precision": precision}
     ctx = mif.Context(gs, **kw)
     ctx.set_stage_records(records)
+    if sweeps is not None:
+        ctx.set_defect_sweeps(sweeps)
+    if schedule is not None:
+        ctx.set_defect_schedule(schedule)
     vel, buf, buf2 = ctx.velocity(), ctx.velocity(), ctx.velocity()
     p, pb = ctx.pressure(), ctx.pressure()
     vel.upload(init[0])
@@ -107,17 +111,20 @@ def _run_n(gs, init, nsteps, fn_kind, precision=None, records=False, multi=True)
 
 @pytest.mark.parametrize("nsteps", [1, 2, 3, 4])
 @pytest.mark.parametrize("ci", [0, 2], ids=["channel", "walls_es"])
-@pytest.mark.parametrize("form", ["fp64", "mixed"])
+@pytest.mark.parametrize("form", ["fp64", "mixed", "mixed_s1", "mixed_s1_last"])
 def test_timestep_n_matches_loop(form, ci, nsteps):
     """mif_timestep_n (each step's last projection in the next step's first
     stage kernel; the pressure ends in p for either parity of the buffer
-    swaps) against the loop of single steps."""
+    swaps) against the loop of single steps; the mixed forms without defect
+    sweeps, with the fused sweep in every solve and in each step's last solve."""
     case = CASES[ci]
     print(f"[case] timestep_n {form} {case[0]} nsteps={nsteps}", flush=True)
     gs, o, init, _, fn = _initial(case)
     prec = None if form == "fp64" else mif.MIXED
-    (nv, np_) = _run_n(gs, init, nsteps, fn, prec)
-    (lv, lp) = _run_n(gs, init, nsteps, fn, prec, multi=False)
+    sweeps = {"fp64": None, "mixed": 0}.get(form, 1)
+    schedule = mif.DEFECT_LAST_STAGE if form == "mixed_s1_last" else None
+    (nv, np_) = _run_n(gs, init, nsteps, fn, prec, sweeps=sweeps, schedule=schedule)
+    (lv, lp) = _run_n(gs, init, nsteps, fn, prec, multi=False, sweeps=sweeps, schedule=schedule)
     tv, tp = (1e-13, 1e-12) if form == "fp64" else (1e-6, 1e-4)
     for c in range(3):
         assert rel_err(nv[c], lv[c]) < tv, c
