@@ -74,6 +74,48 @@ def test_mixed_timestep_channel_error(sweeps, schedule):
     assert ev_all < 1e-6
 
 
+@pytest.mark.parametrize("Ng,per", [((65, 33, 65), (1, 0, 1)), ((33, 33, 65), (0, 0, 1))],
+                         ids=["channel", "walls_x"])
+def test_fused_defect_sweep_solve_tracks_fp64(Ng, per):
+    """The one-sweep form fused into the transform passes (the x forward pass
+    of the correction solve forms the FP64 defect as it loads) against an FP64
+    solve of the same FP32 velocity.  The mixed form stores the pressure in
+    FP32, so the floor is the FP64 solution rounded to FP32: the swept forms
+    (fused one-sweep, unfused two-sweep) reach it, the no-sweep form stays
+    well above it."""
+    print(f"[case] fused defect sweep, stand-alone solve {Ng} {per}", flush=True)
+    og, gs = grids(Ng, (2 * PI, 2.0, PI), (0.0, -1.0, 0.0), 1.0, 1.0, 1, per)
+    dt = 1e-3
+    sols = {}
+    vel32 = None
+    for name, prec, sweeps in (("fp64", mif.FP64, 0), ("s0", mif.MIXED, 0), ("s1", mif.MIXED, 1),
+                               ("s2", mif.MIXED, 2)):
+        ctx = mif.Context(gs, precision=prec)
+        if prec == mif.MIXED:
+            ctx.set_defect_sweeps(sweeps)
+        vel, p = ctx.velocity(), ctx.pressure()
+        if vel32 is None:
+            # random velocity, rounded to FP32 so that every form solves
+            # for the same divergence
+            shapes = [a.shape for a in vel.download()]
+            vel32 = [seeded_normal(sh, 20261210 + c).astype(np.float32).astype(np.float64)
+                     for c, sh in enumerate(shapes)]
+        vel.upload(vel32)
+        mif.solve_pressure_equation_homogeneous_periodic(p, vel, dt)
+        phi = p.download()
+        if name == "fp64":
+            r32 = phi.astype(np.float32).astype(np.float64)
+            sols["fp64_as_fp32"] = r32 - r32.mean()
+        sols[name] = phi - phi.mean()
+        ctx.close()
+    floor = l2rel(sols["fp64_as_fp32"], sols["fp64"])
+    errs = {k: l2rel(sols[k], sols["fp64"]) for k in ("s0", "s1", "s2")}
+    print("errors vs fp64", errs, "FP32 storage floor", floor, flush=True)
+    assert errs["s0"] > 5 * floor
+    assert errs["s1"] < 1.5 * floor
+    assert errs["s2"] < 1.5 * floor
+
+
 def test_defect_schedule_leaves_standalone_solve():
     # the schedule selects sweeps inside mif_timestep only: a stand-alone
     # solve takes the context's sweeps under either schedule, bit for bit
