@@ -287,6 +287,12 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
     }
     for (int q = 0; q < 4; q++) prow[t][q] = rows[q];
   }
+  // DC: each line's owner weight along y and z, formed once per workgroup
+  __shared__ double lwt[DC ? 2 * NP : 1];
+  if (DC && t < NL) {
+    const int r = bx * NL + t;
+    lwt[t] = r < n1 * n2 ? owner_weight(g, 1, r % n1) * owner_weight(g, 2, r / n1) : 0.0;
+  }
   // defect lines (RES): the rows of each line's y and z neighbours
   __shared__ long yzrow[RES ? 2 * NP : 1][4];
   if (RES && t < NL) {
@@ -323,10 +329,7 @@ __global__ void __launch_bounds__(256) k_fft_p2(DGrid g, const T* __restrict__ W
   constexpr int R0 = (L + 255) / 256;  // elements per thread and line on axis 0
   // DC: this thread's owner-weighted sums (W.b or W.phi, W.1)
   double dc_s = 0, dc_w = 0;
-  auto line_weight = [&](int l) -> double {
-    const int r = bx * NL + l;
-    return owner_weight(g, 1, r % n1) * owner_weight(g, 2, r / n1);
-  };
+  auto line_weight = [&](int l) -> double { return lwt[l]; };
   if (AXIS == 0 && FUSED && MODE == 0) {
     const TV* u = (const TV*)io.u;
     const TV* v = (const TV*)io.v;
