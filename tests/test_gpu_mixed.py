@@ -74,8 +74,11 @@ def test_mixed_timestep_channel_error(sweeps, schedule):
     assert ev_all < 1e-6
 
 
-@pytest.mark.parametrize("Ng,per", [((65, 33, 65), (1, 0, 1)), ((33, 33, 65), (0, 0, 1))],
-                         ids=["channel", "walls_x"])
+@pytest.mark.parametrize("Ng,per", [((65, 33, 65), (1, 0, 1)), ((33, 33, 65), (0, 0, 1)),
+                                    # x lines of 512 and 768 points: two and three points per thread of
+                                    # the x passes; 257 Neumann points: a partial second point
+                                    ((513, 9, 17), (1, 0, 1)), ((769, 5, 9), (1, 0, 1)), ((257, 9, 17), (0, 0, 1))],
+                         ids=["channel", "walls_x", "x512", "x768", "x257_walls"])
 def test_fused_defect_sweep_solve_tracks_fp64(Ng, per):
     """The one-sweep form fused into the transform passes (the x forward pass
     of the correction solve forms the FP64 defect as it loads) against an FP64
