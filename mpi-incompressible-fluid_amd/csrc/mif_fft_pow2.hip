@@ -995,6 +995,10 @@ constexpr int np_x(bool d, int L) {
     default: return np_strided(d, L);
   }
 }
+// x inverse passes of length 256: four pairs in FP64 too (256^3 FP64 x
+// inverse + unpack 54.0 -> 49.5 us; the forward + divergence pass measured
+// 110.0 -> 113.5 us with four, so it keeps two)
+constexpr int np_x_inv(bool d, int L) { return L == 256 ? 4 : np_x(d, L); }
 // y pass of length 512 (the 2N-point pass of 257-point Neumann lines): four
 // pairs in FP64 too (256^3 FP64 y pass 71.7 -> 68.4 us; the z pass of length
 // 512 keeps two)
@@ -1116,7 +1120,9 @@ bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, co
       launch_p2<T, 128, np_strided(d, 128)>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
     case 256:
-      if (axis == 0)
+      if (axis == 0 && mode == 1)
+        launch_p2<T, 256, np_x_inv(d, 256), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
+      else if (axis == 0)
         launch_p2<T, 256, np_x(d, 256), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
         launch_p2<T, 256, np_strided(d, 256), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
