@@ -975,7 +975,8 @@ constexpr int np_strided(bool d, int L) {
     case 64: return 16;
     case 128: return d ? 8 : 16;
     case 256: return d ? 4 : 8;
-    case 512: return d ? 2 : 4;
+    // FP64 too: 256^3 y pass 71.7 -> 68.4 us, 512^3 z pass 660 -> 527 us
+    case 512: return 4;
     case 768: return d ? 2 : 4;
     case 1024: return d ? 1 : 2;
     case 1536: return d ? 1 : 2;
@@ -999,10 +1000,6 @@ constexpr int np_x(bool d, int L) {
 // inverse + unpack 54.0 -> 49.5 us; the forward + divergence pass measured
 // 110.0 -> 113.5 us with four, so it keeps two)
 constexpr int np_x_inv(bool d, int L) { return L == 256 ? 4 : np_x(d, L); }
-// y pass of length 512 (the 2N-point pass of 257-point Neumann lines): four
-// pairs in FP64 too (256^3 FP64 y pass 71.7 -> 68.4 us; the z pass of length
-// 512 keeps two)
-constexpr int np_y(bool d, int L) { return L == 512 ? 4 : np_strided(d, L); }
 // the half-length Neumann y pass (k_dct1_y, N = L / 2)
 constexpr int np_dct1(bool d, int N) { return N >= 1024 ? (d ? 1 : 2) : (d ? 2 : 4); }
 
@@ -1130,8 +1127,6 @@ bool launch_fft_pow2(const DGrid& g, int axis, int mode, const FftPlan& plan, co
     case 512:
       if (axis == 0)
         launch_p2<T, 512, np_x(d, 512), 1>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
-      else if (axis == 1)
-        launch_p2<T, 512, np_y(d, 512), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       else
         launch_p2<T, 512, np_strided(d, 512), 2>(g, axis, mode, plan, sc, B, bp0, bplane, io, st);
       return true;
