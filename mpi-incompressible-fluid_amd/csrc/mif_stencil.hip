@@ -936,10 +936,11 @@ __global__ void __launch_bounds__(256) k_unpack(DGrid g, Layout L, const S* __re
 // 79-81) and u -= dt_s * grad(phi) on interior points
 // (update_velocity_pressure_gradient, src/Timestep.cpp:66-76).
 // ---------------------------------------------------------------------------
-// One thread per (i, j) position of the padded plane marches a chunk of k:
-// phi at k-1 stays in a register, and the operands of plane k+1 are loaded
-// while plane k is updated.  Loads read always-valid addresses and the
-// stores carry the point tests, so a plane's loads are issued together.
+// One thread per (i, j) position of the padded plane marches a chunk of k
+// (the launcher's chunk is one plane): phi at k-1 stays in a register, and
+// the operands of plane k+1 are loaded while plane k is updated.  Loads read
+// always-valid addresses and the stores carry the point tests, so a plane's
+// loads are issued together.
 template <typename T, bool NT>
 // The incremented pressure goes to pout: p itself, or another buffer (every
 // pressure point is written).
@@ -1365,8 +1366,10 @@ void launch_project(const DGrid& g, const Layout& L, T* u, T* v, T* w, T* p, con
   if (!pout) pout = p;
   // points 0..N per axis (the union of the extents; u's x+ ghost included,
   // the interior tests filter), as 256-position slices of the padded plane
-  // times chunks of 16 planes
-  constexpr int kchunk = 16;
+  // times one plane per workgroup: the most workgroups in flight (256^3 FP64:
+  // 258 us with chunks of 16 planes, 246 with 8, 235 with 4, 224 with 2, 216
+  // with 1; FP32 112, 108, 104, 99, 99)
+  constexpr int kchunk = 1;
   const int nqt = (int)((L.plane + L.lead + 255) / 256);
   const int nkt = (g.N[2] + 1 + kchunk - 1) / kchunk;
   if (L.nt)
